@@ -1,0 +1,189 @@
+"""ctypes binding of libfmdbzp.so (declared in include/fmdbzp.h).
+
+The library is the only compute path: if it is not built, importing this
+module raises, and every plan creation or search call goes through it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libfmdbzp.so")
+
+ABI_VERSION = 1
+FMDBZP_OK = 0
+
+ELTYPE = {"cf32": 0, "cf64": 1, "i16": 2, "ci8": 3, "ci16": 4, "f32": 5, "f64": 6}
+FLAG_STORE_SURFACES = 0x1
+FLAG_DEVICE_SIGNAL = 0x2
+FLAG_DEVICE_OUTPUT = 0x4
+
+KERNEL_CLASSES = ("code_fft", "signal_fft", "corr_tile", "column", "reduce", "finalize", "export")
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("samples_per_code", ctypes.c_int32),
+        ("num_blocks", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("fft_size", ctypes.c_int32),
+        ("num_coh", ctypes.c_int32),
+        ("num_data_bits", ctypes.c_int32),
+        ("bit_edge_search_steps", ctypes.c_int32),
+        ("num_noncoherent", ctypes.c_int32),
+        ("num_secondary_rotations", ctypes.c_int32),
+        ("num_doppler_bins", ctypes.c_int32),
+        ("samples_per_code_eff", ctypes.c_int32),
+        ("num_patterns", ctypes.c_int32),
+        ("path", ctypes.c_int32),
+        ("noise_estimator", ctypes.c_int32),
+        ("samples_per_chip", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+        ("sampling_freq_hz", ctypes.c_double),
+    ]
+
+
+class PrnStats(ctypes.Structure):
+    _fields_ = [
+        ("peak_power", ctypes.c_float),
+        ("peak_row", ctypes.c_int32),
+        ("peak_col", ctypes.c_int32),
+        ("noise_power", ctypes.c_float),
+        ("code_left", ctypes.c_float),
+        ("code_right", ctypes.c_float),
+        ("dop_left", ctypes.c_float),
+        ("dop_right", ctypes.c_float),
+        ("noise_sum", ctypes.c_double),
+        ("noise_cells", ctypes.c_int64),
+    ]
+
+
+EXPORTED_SYMBOLS = (
+    "fmdbzp_plan_create",
+    "fmdbzp_acquire",
+    "fmdbzp_plan_destroy",
+    "fmdbzp_last_error",
+    "fmdbzp_query",
+    "fmdbzp_plan_set_timing",
+    "fmdbzp_plan_kernel_times",
+)
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"libfmdbzp.so is not built ({path}); run __graft_entry__.build() "
+                          "or `make -C acquisition.jl_amd/csrc`")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    lib.fmdbzp_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.c_int32, P, ctypes.c_int32, P, P, P,
+                                       ctypes.POINTER(ctypes.c_void_p)]
+    lib.fmdbzp_plan_create.restype = ctypes.c_int
+    lib.fmdbzp_acquire.argtypes = [P, P, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, P, ctypes.c_int32,
+                                   ctypes.c_uint32, P, P]
+    lib.fmdbzp_acquire.restype = ctypes.c_int
+    lib.fmdbzp_plan_destroy.argtypes = [P]
+    lib.fmdbzp_plan_destroy.restype = ctypes.c_int
+    lib.fmdbzp_last_error.argtypes = []
+    lib.fmdbzp_last_error.restype = ctypes.c_char_p
+    lib.fmdbzp_query.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.fmdbzp_query.restype = ctypes.c_int
+    lib.fmdbzp_plan_set_timing.argtypes = [P, ctypes.c_int32]
+    lib.fmdbzp_plan_set_timing.restype = ctypes.c_int
+    lib.fmdbzp_plan_kernel_times.argtypes = [P, P, P, ctypes.c_int32]
+    lib.fmdbzp_plan_kernel_times.restype = ctypes.c_int
+    return lib
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+class EngineError(RuntimeError):
+    """A negative status from libfmdbzp (HIP error, OOM, no device, bad handle)."""
+
+
+def _check(rc: int, what: str):
+    if rc != FMDBZP_OK:
+        msg = lib().fmdbzp_last_error().decode(errors="replace")
+        raise EngineError(f"{what} failed (status {rc}): {msg}")
+
+
+def query() -> str:
+    buf = ctypes.create_string_buffer(512)
+    rc = lib().fmdbzp_query(buf, len(buf))
+    _check(rc, "fmdbzp_query")
+    return buf.value.decode()
+
+
+def make_desc(fields: dict) -> PlanDesc:
+    d = PlanDesc()
+    d.abi_version = ABI_VERSION
+    for k, v in fields.items():
+        setattr(d, k, v)
+    return d
+
+
+class NativePlan:
+    """Owner of one `fmdbzp_plan*`."""
+
+    def __init__(self, handle, desc: PlanDesc, nprn: int):
+        self.handle = handle
+        self.desc = desc
+        self.nprn = nprn
+
+    @classmethod
+    def create(cls, plan) -> "NativePlan":
+        from .plan import plan_code_samples, plan_desc_fields, plan_patterns
+
+        desc = make_desc(plan_desc_fields(plan))
+        codes = np.ascontiguousarray(plan_code_samples(plan), dtype=np.int8)
+        nsets, pats, pset = plan_patterns(plan)
+        pats = np.ascontiguousarray(pats, dtype=np.float32)
+        pset = np.ascontiguousarray(pset, dtype=np.int32)
+        shifts = np.ascontiguousarray(plan.code_drift_shifts, dtype=np.int32)
+        h = ctypes.c_void_p()
+        rc = lib().fmdbzp_plan_create(ctypes.byref(desc), len(plan.avail_prns), codes.ctypes.data, nsets,
+                                      pats.ctypes.data, pset.ctypes.data, shifts.ctypes.data, ctypes.byref(h))
+        _check(rc, "fmdbzp_plan_create")
+        return cls(h, desc, len(plan.avail_prns))
+
+    def acquire(self, signal_ptr: int, eltype: int, n_samples: int, if_hz: float, prn_index, flags: int,
+                surfaces=None):
+        idx = np.ascontiguousarray(prn_index, dtype=np.int32)
+        stats = (PrnStats * len(idx))()
+        surf_arg = None
+        if surfaces is not None:
+            arr = (ctypes.c_void_p * len(idx))(*[ctypes.c_void_p(int(s)) if s else None for s in surfaces])
+            surf_arg = arr
+        rc = lib().fmdbzp_acquire(self.handle, ctypes.c_void_p(signal_ptr), eltype, n_samples, float(if_hz),
+                                  idx.ctypes.data, len(idx), flags, ctypes.cast(stats, ctypes.c_void_p),
+                                  ctypes.cast(surf_arg, ctypes.c_void_p) if surf_arg is not None else None)
+        _check(rc, "fmdbzp_acquire")
+        return stats
+
+    def set_timing(self, enable: bool):
+        _check(lib().fmdbzp_plan_set_timing(self.handle, 1 if enable else 0), "fmdbzp_plan_set_timing")
+
+    def kernel_times(self) -> dict:
+        n = len(KERNEL_CLASSES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
+        lib().fmdbzp_plan_kernel_times(self.handle, ctypes.cast(ms, ctypes.c_void_p),
+                                       ctypes.cast(cnt, ctypes.c_void_p), n)
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(KERNEL_CLASSES)}
+
+    def destroy(self):
+        if self.handle:
+            lib().fmdbzp_plan_destroy(self.handle)
+            self.handle = None

@@ -1,0 +1,329 @@
+"""`acquire_` / `acquire`: run the engine and assemble `AcquisitionResults`.
+
+Mirrors src/acquire.jl:54-882 and src/Acquisition.jl:81-142. The search
+(downconversion through per-PRN peak, noise and neighbour cells) runs in
+libfmdbzp on the GPU; this module validates arguments with the reference's
+messages, and evaluates the reference's scalar result formulas with its exact
+Float32/Float64 mix (peak-to-noise, CN0, code phase, parabolic interpolation,
+the secondary-code phase gate and estimator).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Any, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _native
+from .cfar import cfar_threshold
+from .plan import AcquisitionPlan, ArgumentError, plan_acquire
+
+f32 = np.float32
+
+
+@dataclass
+class AcquisitionResults:
+    """Results of one PRN (src/Acquisition.jl:81-109). Frequencies in Hz."""
+
+    system: Any
+    prn: int
+    sampling_frequency: float
+    carrier_doppler: float
+    code_phase: float
+    secondary_code_phase: Optional[int]
+    CN0: float
+    noise_power: np.float32
+    peak_to_noise_ratio: np.float32
+    num_noncoherent_integrations: int
+    power_bins: Optional[np.ndarray]
+    dopplers: np.ndarray
+    num_blocks: int
+    block_size: int
+    num_secondary_rotations: int
+
+    def __repr__(self):
+        return (f"AcquisitionResults: PRN {self.prn}, CN0 = {round(self.CN0, 2)} dB-Hz, "
+                f"Doppler = {self.carrier_doppler} Hz, Code phase = {round(self.code_phase, 3)} chips")
+
+
+def get_num_cells(result: AcquisitionResults) -> int:
+    """Search cells: Doppler bins x samples per code x secondary rotations
+    (src/Acquisition.jl:124-126)."""
+    return len(result.dopplers) * result.num_blocks * result.block_size * result.num_secondary_rotations
+
+
+def is_detected(result: AcquisitionResults, pfa: float = 0.01) -> bool:
+    """peak_to_noise_ratio > cfar_threshold(pfa, get_num_cells) (src/Acquisition.jl:135-142)."""
+    thr = cfar_threshold(pfa, get_num_cells(result),
+                         num_noncoherent_integrations=result.num_noncoherent_integrations)
+    return bool(float(result.peak_to_noise_ratio) > thr)
+
+
+def fmdbzp_column_to_tau(scrambled_col: int, num_blocks: int, block_size: int) -> int:
+    """Scrambled FM-DBZP column (0-based) -> delay in samples (src/plot.jl:16-20)."""
+    block_row = scrambled_col // block_size
+    within = scrambled_col % block_size
+    return ((num_blocks - block_row) % num_blocks) * block_size + within
+
+
+def jl_mod(x: float, y: float) -> float:
+    """Julia `mod` for Float64: fmod, shifted into the sign of y."""
+    r = math.fmod(x, y)
+    if r == 0.0:
+        return math.copysign(0.0, y)
+    if (r > 0) != (y > 0):
+        return r + y
+    return r
+
+
+def parabolic_interp(left, peak, right):
+    """`_parabolic_interp` (src/acquire.jl:9-13) on Float32 operands: returns a
+    float32 offset, or the Float64 0.0 the reference returns for a zero
+    denominator."""
+    left, peak, right = f32(left), f32(peak), f32(right)
+    denom = f32(2) * f32(f32(f32(f32(2) * peak) - left) - right)
+    if denom == 0:
+        return 0.0
+    return f32(f32(right - left) / denom)
+
+
+# ---------------------------------------------------------------------------
+# signal views
+# ---------------------------------------------------------------------------
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch") and hasattr(x, "data_ptr")
+
+
+def _numpy_eltype(a: np.ndarray):
+    if a.dtype == np.complex64 and a.ndim == 1:
+        return "cf32", a.shape[0]
+    if a.dtype == np.complex128 and a.ndim == 1:
+        return "cf64", a.shape[0]
+    if a.dtype == np.int16 and a.ndim == 1:
+        return "i16", a.shape[0]
+    if a.dtype == np.int16 and a.ndim == 2 and a.shape[1] == 2:
+        return "ci16", a.shape[0]
+    if a.dtype == np.int8 and a.ndim == 2 and a.shape[1] == 2:
+        return "ci8", a.shape[0]
+    if a.dtype == np.float32 and a.ndim == 1:
+        return "f32", a.shape[0]
+    if a.dtype == np.float64 and a.ndim == 1:
+        return "f64", a.shape[0]
+    raise ArgumentError(f"unsupported signal array dtype {a.dtype} with shape {a.shape}")
+
+
+class _SignalView:
+    """Pointer, sample format and length of a host or device signal."""
+
+    def __init__(self, signal):
+        self.obj = signal
+        if _is_torch(signal):
+            import torch
+
+            t = signal
+            if not t.is_contiguous():
+                raise ArgumentError("device signal must be contiguous")
+            m = {torch.complex64: "cf32", torch.complex128: "cf64", torch.float32: "f32", torch.float64: "f64"}
+            if t.dtype in m and t.dim() == 1:
+                self.kind, self.n = m[t.dtype], t.shape[0]
+            elif t.dtype == torch.int16 and t.dim() == 1:
+                self.kind, self.n = "i16", t.shape[0]
+            elif t.dtype == torch.int16 and t.dim() == 2 and t.shape[1] == 2:
+                self.kind, self.n = "ci16", t.shape[0]
+            elif t.dtype == torch.int8 and t.dim() == 2 and t.shape[1] == 2:
+                self.kind, self.n = "ci8", t.shape[0]
+            else:
+                raise ArgumentError(f"unsupported signal tensor dtype {t.dtype} with shape {tuple(t.shape)}")
+            self.device = t.is_cuda
+            if self.device:
+                torch.cuda.current_stream(t.device).synchronize()
+            else:
+                self.obj = np.ascontiguousarray(t.numpy())
+            self.ptr = t.data_ptr() if self.device else self.obj.ctypes.data
+        else:
+            a = np.ascontiguousarray(signal)
+            self.obj = a
+            self.kind, self.n = _numpy_eltype(a)
+            self.device = False
+            self.ptr = a.ctypes.data
+
+    def complex64_prefix(self, n: int) -> np.ndarray:
+        """ComplexF32 of the first n samples on the host."""
+        x = self.obj[:n]
+        if _is_torch(x):
+            x = x.cpu().numpy()
+        if self.kind in ("ci8", "ci16"):
+            return (x[:, 0].astype(np.float32) + 1j * x[:, 1].astype(np.float32)).astype(np.complex64)
+        return np.asarray(x).astype(np.complex64)
+
+
+# ---------------------------------------------------------------------------
+# secondary-code phase (host, on the raw signal)
+# ---------------------------------------------------------------------------
+def _estimate_secondary_code_phase(plan: AcquisitionPlan, prn: int, sig: np.ndarray, if_hz: float,
+                                   fs: float, code_phase: float, doppler_hz: float) -> int:
+    """`_estimate_secondary_code_phase` (src/acquire.jl:503-538): despread the
+    first L periods at the recovered (Doppler, code phase) and take the
+    strongest of the L secondary-code rotations."""
+    system = plan.system
+    L = plan.num_secondary_rotations
+    spc = plan.samples_per_code
+    code = system.gen_code(spc, prn, fs, system.get_code_frequency(), code_phase).astype(np.float32)
+    phase_step = -2.0 * math.pi * (if_hz + doppler_hz) / fs
+    dphi = np.complex64(complex(f32(math.cos(phase_step)), f32(math.sin(phase_step))))
+    zs = []
+    for k in range(L):
+        base = k * spc
+        a = phase_step * base
+        seq = np.empty(spc, dtype=np.complex64)
+        seq[0] = np.complex64(complex(f32(math.cos(a)), f32(math.sin(a))))
+        seq[1:] = dphi
+        carr = np.cumprod(seq, dtype=np.complex64)
+        zs.append(np.complex64(np.sum((sig[base:base + spc] * carr) * code, dtype=np.complex64)))
+    best_r, best = 0, f32(-1.0)
+    for r in range(L):
+        s = np.complex64(0)
+        for k in range(L):
+            s = np.complex64(s + np.complex64(f32(system.secondary_value(prn, (k + r) % L)) * zs[k]))
+        mag = f32(f32(s.real * s.real) + f32(s.imag * s.imag))
+        if mag > best:
+            best, best_r = mag, r
+    return best_r
+
+
+# ---------------------------------------------------------------------------
+# acquire_
+# ---------------------------------------------------------------------------
+def _extract(plan: AcquisitionPlan, st, prn: int, sigview: _SignalView, if_hz: float,
+             subsample_interpolation: bool, power_bins) -> AcquisitionResults:
+    """Result extraction (src/acquire.jl:544-671 and :681-781) from the engine's
+    per-PRN statistics."""
+    system = plan.system
+    fs = plan.sampling_freq
+    fc = system.get_code_frequency()
+    code_length = system.get_code_length()
+    ndop = plan.num_doppler_bins
+    spc = plan.samples_per_code
+    peak = f32(st.peak_power)
+    noise = f32(st.noise_power)
+    signal_power = f32(peak - noise)
+    ptn = f32(f32(signal_power + noise) / noise)
+    code_period = code_length / fc
+    tcoh = plan.num_coherently_integrated_code_periods * code_period
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = float(f32(signal_power / noise))
+    cn0 = 10 * math.log10(ratio / tcoh / 1.0) if ratio > 0 else (-math.inf if ratio == 0 else math.nan)
+
+    row, col = int(st.peak_row), int(st.peak_col)
+    rotation_block = col // spc
+    scrambled = col % spc
+    delay = fmdbzp_column_to_tau(scrambled, plan.num_blocks, plan.block_size)
+    code_phase = jl_mod(-delay * fc / fs, code_length)
+    doppler = float(plan.doppler_freqs[row])
+    if subsample_interpolation:
+        pl, pr = f32(st.code_left), f32(st.code_right)
+        dl, dr = f32(st.dop_left), f32(st.dop_right)
+        sq = np.sqrt(noise)
+        if max(pl, pr) > sq:
+            frac = parabolic_interp(pl, peak, pr)
+            if isinstance(frac, np.float32):
+                di = f32(f32(delay) + frac)
+                code_phase = jl_mod(float(-di) * fc / fs, code_length)
+        if max(dl, dr) > sq:
+            frac = parabolic_interp(dl, peak, dr)
+            doppler = doppler + float(frac) * plan.doppler_spacing_hz
+
+    secondary_code_phase = None
+    if plan.num_secondary_rotations > 1:
+        num_cells = ndop * plan.samples_per_code * plan.num_secondary_rotations
+        thr = cfar_threshold(0.01, num_cells, num_noncoherent_integrations=plan.num_noncoherent_accumulations)
+        if float(ptn) > thr:
+            n = plan.num_secondary_rotations * spc
+            sig = sigview.complex64_prefix(n)
+            secondary_code_phase = _estimate_secondary_code_phase(plan, prn, sig, if_hz, fs, code_phase, doppler)
+
+    return AcquisitionResults(
+        system=system, prn=prn, sampling_frequency=fs, carrier_doppler=doppler, code_phase=code_phase,
+        secondary_code_phase=secondary_code_phase, CN0=cn0, noise_power=noise, peak_to_noise_ratio=ptn,
+        num_noncoherent_integrations=plan.num_noncoherent_accumulations, power_bins=power_bins,
+        dopplers=plan.doppler_freqs, num_blocks=plan.num_blocks, block_size=plan.block_size,
+        num_secondary_rotations=plan.num_secondary_rotations)
+
+
+def _get_result_buffer(plan: AcquisitionPlan, prn_idx: int) -> np.ndarray:
+    """Plan-owned (ndop, spc_eff) Float32 surface, allocated on first use and
+    reused (src/acquire.jl:221-228)."""
+    buf = plan.result_buffers[prn_idx]
+    if buf is None:
+        buf = np.empty((plan.num_doppler_bins, plan.samples_per_code_eff), dtype=np.float32, order="F")
+        plan.result_buffers[prn_idx] = buf
+    return buf
+
+
+def acquire_(plan: AcquisitionPlan, signal, prns: Union[int, Sequence[int]], *, interm_freq: float = 0.0,
+             subsample_interpolation: bool = False, store_power_bins: bool = False):
+    """FM-DBZP acquisition with a pre-computed plan (`acquire!`, src/acquire.jl:54-89).
+
+    `signal` is a 1-D numpy array (complex64, complex128, int16 real,
+    float32/float64 real), an (n, 2) int8/int16 array of complex integer
+    samples, or the same as a torch tensor (a CUDA tensor is read in place).
+    Returns one `AcquisitionResults` per PRN (a single result for an int).
+    """
+    if isinstance(prns, (int, np.integer)):
+        return acquire_(plan, signal, [int(prns)], interm_freq=interm_freq,
+                        subsample_interpolation=subsample_interpolation, store_power_bins=store_power_bins)[0]
+    prns = [int(p) for p in prns]
+    if not all(p in plan.avail_prns for p in prns):
+        raise ArgumentError(f"All requested PRNs must be in plan.avail_prns. Got: {prns}, "
+                            f"available: {plan.avail_prns}")
+    if plan.native is None:
+        raise _native.EngineError("plan has no device plan (closed, or built without plan_acquire)")
+    sv = _SignalView(signal)
+    segment_length = plan.num_coherently_integrated_code_periods * plan.samples_per_code
+    num_segments = sv.n // segment_length
+    if num_segments < plan.num_noncoherent_accumulations:
+        raise ArgumentError(
+            f"Signal has {sv.n} samples → {num_segments} full segments of {segment_length}, "
+            f"but plan.num_noncoherent_accumulations={plan.num_noncoherent_accumulations}. "
+            "Provide a longer signal.")
+    if not prns:
+        plan.acq_results_buf = []
+        return plan.acq_results_buf
+    idx = [plan.avail_prns.index(p) for p in prns]
+    flags = _native.FLAG_DEVICE_SIGNAL if sv.device else 0
+    surfaces = None
+    bufs = None
+    if store_power_bins:
+        flags |= _native.FLAG_STORE_SURFACES
+        bufs = [_get_result_buffer(plan, i) for i in idx]
+        surfaces = [b.ctypes.data for b in bufs]
+    stats = plan.native.acquire(sv.ptr, _native.ELTYPE[sv.kind], sv.n, float(interm_freq), idx, flags, surfaces)
+    results = []
+    for j, p in enumerate(prns):
+        results.append(_extract(plan, stats[j], p, sv, float(interm_freq), subsample_interpolation,
+                                bufs[j] if bufs is not None else None))
+    plan.acq_results_buf = results
+    return results
+
+
+def acquire(system, signal, sampling_freq: float, prns, *, interm_freq: float = 0.0,
+            min_doppler_coverage: float = 7000.0, num_coherently_integrated_code_periods: int = 1,
+            bit_edge_search_steps: int = 1, num_noncoherent_accumulations: int = 1,
+            use_secondary_code: bool = True, max_secondary_code_rotations: int = 32, fft_flag=None,
+            subsample_interpolation: bool = False, store_power_bins: bool = False, **kw):
+    """`plan_acquire` then `acquire_` (src/acquire.jl:838-882)."""
+    single = isinstance(prns, (int, np.integer))
+    plist = [int(prns)] if single else [int(p) for p in prns]
+    plan = plan_acquire(system, sampling_freq, plist, min_doppler_coverage=min_doppler_coverage,
+                        num_coherently_integrated_code_periods=num_coherently_integrated_code_periods,
+                        bit_edge_search_steps=bit_edge_search_steps,
+                        num_noncoherent_accumulations=num_noncoherent_accumulations,
+                        use_secondary_code=use_secondary_code,
+                        max_secondary_code_rotations=max_secondary_code_rotations, **kw)
+    try:
+        res = acquire_(plan, signal, plist, interm_freq=interm_freq,
+                       subsample_interpolation=subsample_interpolation, store_power_bins=store_power_bins)
+    finally:
+        plan.close()
+    return res[0] if single else res

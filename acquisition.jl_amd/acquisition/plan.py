@@ -1,0 +1,378 @@
+"""`plan_acquire`: geometry, validation and the device-resident plan.
+
+Mirrors src/plan.jl:312-697 of Acquisition.jl. Every scalar of the search
+geometry is derived here with the reference's exact arithmetic (Python floats
+are IEEE doubles, evaluated in the reference's operation order); the device
+plan then receives them through the C ABI (`fmdbzp_plan_create`) together
+with the code samples, sign patterns and per-step code-drift shifts.
+"""
+from __future__ import annotations
+
+import math
+from fractions import Fraction
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .estimators import AbstractNoiseEstimator, GlobalMeanNoiseEstimator, OppositeRowNoiseEstimator
+from .systems import GNSSSystem, samples_per_code as _samples_per_code
+
+
+class ArgumentError(ValueError):
+    """Julia's `ArgumentError`: raised for invalid arguments before any work."""
+
+
+# Crossover of the reference's batched column FFT (src/plan.jl:7). The engine
+# has a single column kernel; the constant is kept for API parity.
+BATCH_FFT_THRESHOLD = 320
+
+PATH_SIMPLE, PATH_SIGN, PATH_ROTATION = 0, 1, 2
+
+
+def nextprod(factors: Sequence[int], n: int) -> int:
+    """Smallest integer >= n whose prime factors are all in `factors`."""
+    m = max(int(n), 1)
+    while True:
+        k = m
+        for f in factors:
+            while k % f == 0:
+                k //= f
+        if k == 1:
+            return m
+        m += 1
+
+
+def _jl_rat(x: float):
+    """Julia `Base.rat`: continued-fraction rational (a, b) of a Float64,
+    stopping when a/b == x or the terms exceed maxintfloat(Float32)."""
+    y = x
+    a, d = 1, 1
+    b, c = 0, 0
+    m = 16777216.0  # maxintfloat(narrow(Float64)) = maxintfloat(Float32)
+    while abs(y) <= m:
+        f = int(math.trunc(y))
+        y -= f
+        a, c = f * a + c, a
+        b, d = f * b + d, b
+        if max(abs(a), abs(b)) > int(m):
+            return c, d
+        if b != 0 and float(a) / float(b) == x:
+            break
+        if y == 0:
+            break
+        y = 1.0 / y
+    return a, b
+
+
+def julia_float_range(start: float, step: float, length: int) -> np.ndarray:
+    """Values of Julia's `range(start; step, length)` for Float64 arguments.
+
+    Julia first tries exact rationals for start and step (`rat`); when both
+    round-trip it evaluates (start_n + i step_n) / den exactly and rounds once,
+    otherwise it evaluates start + i step in extended precision and rounds.
+    Both are reproduced with exact rational arithmetic here.
+    """
+    sn, sd = _jl_rat(start)
+    tn, td = _jl_rat(step)
+    if sd != 0 and td != 0 and float(sn) / float(sd) == start and float(tn) / float(td) == step:
+        fs = Fraction(sn, sd)
+        ft = Fraction(tn, td)
+    else:
+        fs = Fraction(start)
+        ft = Fraction(step)
+    return np.array([float(fs + i * ft) for i in range(length)], dtype=np.float64)
+
+
+def sign_patterns(system: Optional[GNSSSystem], prn: int, num_data_bits: int,
+                  num_secondary_rotations: int, num_coh_periods: int,
+                  use_secondary_code: bool) -> np.ndarray:
+    """`sign_patterns` (src/sign_patterns.jl:23-47): (N_coh, Q) Float32 +-1
+    patterns, rotation-major and data-bit-minor."""
+    num_data_combos = 1 << (num_data_bits - 1)
+    sec_active = use_secondary_code and num_secondary_rotations > 1
+    num_sec = num_secondary_rotations if sec_active else 1
+    coh_per_bit = num_coh_periods // num_data_bits
+    L = num_secondary_rotations if sec_active else 1
+    out = np.empty((num_coh_periods, num_data_combos * num_sec), dtype=np.float32)
+    col = 0
+    for r in range(num_sec):
+        for d in range(num_data_combos):
+            for k in range(num_coh_periods):
+                bit = k // coh_per_bit
+                ds = 1.0 if bit == 0 else (1.0 if ((d >> (bit - 1)) & 1) == 0 else -1.0)
+                ss = float(system.secondary_value(prn, (k + r) % L)) if sec_active else 1.0
+                out[k, col] = np.float32(ds * ss)
+            col += 1
+    return out
+
+
+class AcquisitionPlan:
+    """Pre-computed FM-DBZP acquisition plan (Heckler & Garrison 2009).
+
+    Public fields keep the meaning of the reference's `AcquisitionPlan`
+    (src/plan.jl:81-174): `samples_per_code`, `samples_per_code_eff`,
+    `num_blocks`, `block_size`, `num_coherently_integrated_code_periods`,
+    `num_data_bits`, `bit_edge_search_steps`, `num_noncoherent_accumulations`,
+    `doppler_freqs`, `fftshift_perm`, `result_buffers`, `avail_prns`,
+    `noise_estimator`, `use_secondary_code`, `max_secondary_code_rotations`,
+    `num_secondary_rotations`, `sign_patterns_by_prn`. The FFT caches and
+    scratch live on the device behind `native`.
+    """
+
+    def __init__(self):
+        self.native = None
+
+    @property
+    def num_doppler_bins(self) -> int:
+        return len(self.doppler_freqs)
+
+    @property
+    def doppler_step(self) -> float:
+        return self.doppler_spacing_hz
+
+    def close(self):
+        if self.native is not None:
+            self.native.destroy()
+            self.native = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _plan_host(system: GNSSSystem, sampling_freq: float, prns: Sequence[int], *,
+               min_doppler_coverage: float = 7000.0,
+               num_coherently_integrated_code_periods: int = 1,
+               bit_edge_search_steps: int = 1,
+               num_noncoherent_accumulations: int = 1,
+               noise_estimator: AbstractNoiseEstimator = OppositeRowNoiseEstimator(),
+               use_secondary_code: bool = True,
+               max_secondary_code_rotations: int = 32) -> AcquisitionPlan:
+    """Host half of `plan_acquire` (src/plan.jl:325-672): validation, geometry,
+    Doppler grid, sign patterns, drift shifts. No device work."""
+    N = num_coherently_integrated_code_periods
+    E = bit_edge_search_steps
+    Nnc = num_noncoherent_accumulations
+    if not Nnc >= 1:
+        raise ArgumentError(f"num_noncoherent_accumulations must be >= 1, got {Nnc}")
+    if not N >= 1:
+        raise ArgumentError("num_coherently_integrated_code_periods must be >= 1")
+    if not E >= 1:
+        raise ArgumentError(f"bit_edge_search_steps must be >= 1, got {E}")
+    if N == 1 and E > 1:
+        raise ArgumentError(
+            "bit_edge_search_steps must be 1 when num_coherently_integrated_code_periods == 1 "
+            f"(sub-bit mode; no bit boundary search applies), got {E}")
+
+    L = system.get_secondary_code_length()
+    if use_secondary_code and L > 1 and N > 1 and L > max_secondary_code_rotations:
+        raise ArgumentError(
+            "Secondary-code rotation search size exceeds the per-plan cap for "
+            f"{system.name}: "
+            f"get_secondary_code_length(system)={L} > max_secondary_code_rotations={max_secondary_code_rotations} "
+            f"at num_coherently_integrated_code_periods={N}. "
+            "Remedies (in order): "
+            "(1) reduce num_coherently_integrated_code_periods; "
+            f"(2) raise max_secondary_code_rotations to at least {L}; or "
+            "(3) set use_secondary_code = false to opt out of the rotation search.")
+    if use_secondary_code and L > 1 and N > 1 and N % L != 0:
+        raise ArgumentError(
+            "Secondary-code rotation search requires num_coherently_integrated_code_periods "
+            f"to be a whole multiple of the secondary-code length L={L}, got "
+            f"num_coherently_integrated_code_periods={N}. "
+            "A partial secondary-code period makes the rotation search produce a ±Doppler "
+            "sign ambiguity — a near-equal mirror peak at -f whose sign flips at random under "
+            "noise (issue #68). Remedies (in order): "
+            f"(1) set num_coherently_integrated_code_periods to a multiple of {L} "
+            f"(e.g. {L} for one full secondary-code period); "
+            "(2) set num_coherently_integrated_code_periods = 1 and raise "
+            "num_noncoherent_accumulations for a robust sign at coarser Doppler resolution; or "
+            "(3) set use_secondary_code = false to opt out of the rotation search.")
+
+    fs = float(sampling_freq)
+    spc = _samples_per_code(system, fs)
+
+    data_freq = system.get_data_frequency()
+    has_data = math.isfinite(data_freq) and data_freq > 0
+    if has_data:
+        bit_samples = math.ceil(fs / data_freq)
+        bit_period_codes = bit_samples // spc
+    else:
+        bit_period_codes = N
+
+    bin_width = fs / spc
+    cov = float(min_doppler_coverage)
+    min_num_blocks = math.ceil(2 * cov / bin_width + 2 / N)
+    num_blocks = None
+    for cand in range(max(min_num_blocks, 1), spc + 1):
+        if spc % cand == 0:
+            num_blocks = cand
+            break
+    if min_num_blocks > spc:
+        num_blocks = None
+    if num_blocks is None:
+        raise ArgumentError(
+            f"No valid num_blocks found: samples_per_code={spc} has no divisor "
+            f">= min_num_blocks={min_num_blocks}. "
+            "This should not happen for standard GNSS sampling frequencies — please file a bug.")
+
+    if has_data and N >= bit_period_codes:
+        if N % bit_period_codes != 0:
+            raise ArgumentError(
+                f"num_coherently_integrated_code_periods={N} must be divisible by "
+                f"bit_period_codes={bit_period_codes} (code periods per data bit).")
+        if bit_period_codes % E != 0:
+            raise ArgumentError(
+                f"bit_edge_search_steps={E} must divide bit_period_codes={bit_period_codes}.")
+
+    block_size = spc // num_blocks
+    fft_size = nextprod((2, 3, 5, 7), 2 * block_size)
+    num_data_bits = N // bit_period_codes if (has_data and N >= bit_period_codes) else 1
+    ndop = N * num_blocks
+
+    doppler_coverage_hz = num_blocks * fs / spc
+    spacing = doppler_coverage_hz / ndop
+    doppler_freqs = julia_float_range(-doppler_coverage_hz / 2, spacing, ndop)
+
+    rotation_active = use_secondary_code and L > 1 and N > 1
+    R = L if rotation_active else 1
+    spc_eff = spc * R if rotation_active else spc
+    sign_path = num_data_bits > 1 or E > 1 or rotation_active
+    path = PATH_ROTATION if rotation_active else (PATH_SIGN if sign_path else PATH_SIMPLE)
+
+    prns = [int(p) for p in prns]
+    plan = AcquisitionPlan()
+    plan.system = system
+    plan.sampling_freq = fs
+    plan.samples_per_code = spc
+    plan.samples_per_code_eff = spc_eff
+    plan.num_blocks = num_blocks
+    plan.block_size = block_size
+    plan.fft_size = fft_size
+    plan.num_coherently_integrated_code_periods = N
+    plan.num_data_bits = num_data_bits
+    plan.bit_edge_search_steps = E
+    plan.num_noncoherent_accumulations = Nnc
+    plan.bit_period_codes = bit_period_codes
+    plan.doppler_freqs = doppler_freqs
+    plan.doppler_coverage_hz = doppler_coverage_hz
+    plan.doppler_spacing_hz = spacing
+    plan.fftshift_perm = np.array([(r - 1 + ndop // 2) % ndop + 1 for r in range(1, ndop + 1)], dtype=np.int64)
+    plan.result_buffers = [None for _ in prns]
+    plan.avail_prns = prns
+    plan.noise_estimator = noise_estimator
+    plan.use_secondary_code = bool(use_secondary_code)
+    plan.max_secondary_code_rotations = int(max_secondary_code_rotations)
+    plan.num_secondary_rotations = R
+    plan.path = path
+    plan.acq_results_buf = []
+
+    # sign patterns (src/plan.jl:627-672)
+    plan.sign_patterns_by_prn = {}
+    if sign_path:
+        prn_independent = (not rotation_active) or (not system.is_per_prn_secondary())
+        if prn_independent:
+            shared = sign_patterns(system if use_secondary_code else None, prns[0], num_data_bits, R, N,
+                                   use_secondary_code)
+            for p in prns:
+                plan.sign_patterns_by_prn[p] = shared
+        else:
+            for p in prns:
+                plan.sign_patterns_by_prn[p] = sign_patterns(system, p, num_data_bits, R, N, use_secondary_code)
+    plan.num_patterns = (1 << (num_data_bits - 1)) * R if sign_path else 1
+
+    # code-drift shifts per step, sorted-row keyed (src/noncoherent_integration.jl:600-619)
+    carrier = system.get_center_frequency()
+    coherent_duration_s = (N * spc) / fs
+    shifts = np.zeros((Nnc, ndop), dtype=np.int32)
+    for m in range(Nnc):
+        if m == 0:
+            continue
+        for r in range(ndop):
+            raw = round(float(doppler_freqs[r]) * m * coherent_duration_s * fs / carrier)
+            shifts[m, r] = raw % spc
+    plan.code_drift_shifts = shifts
+
+    est = noise_estimator
+    plan.noise_estimator_id = 1 if isinstance(est, GlobalMeanNoiseEstimator) else 0
+    plan.samples_per_chip = math.floor(fs / system.get_code_frequency())
+    return plan
+
+
+def plan_acquire(system: GNSSSystem, sampling_freq: float, prns: Sequence[int], *,
+                 min_doppler_coverage: float = 7000.0,
+                 num_coherently_integrated_code_periods: int = 1,
+                 bit_edge_search_steps: int = 1,
+                 num_noncoherent_accumulations: int = 1,
+                 noise_estimator: AbstractNoiseEstimator = OppositeRowNoiseEstimator(),
+                 use_secondary_code: bool = True,
+                 max_secondary_code_rotations: int = 32,
+                 fft_flag=None) -> AcquisitionPlan:
+    """Pre-compute an FM-DBZP acquisition plan (src/plan.jl:312-324).
+
+    Keyword arguments and validation errors are the reference's; frequencies
+    are plain Hz floats. `fft_flag` (FFTW planning) has no effect: the engine's
+    transforms are fixed kernels. The plan owns device memory until `close()`.
+    """
+    from . import _native
+
+    plan = _plan_host(system, sampling_freq, prns,
+                      min_doppler_coverage=min_doppler_coverage,
+                      num_coherently_integrated_code_periods=num_coherently_integrated_code_periods,
+                      bit_edge_search_steps=bit_edge_search_steps,
+                      num_noncoherent_accumulations=num_noncoherent_accumulations,
+                      noise_estimator=noise_estimator,
+                      use_secondary_code=use_secondary_code,
+                      max_secondary_code_rotations=max_secondary_code_rotations)
+    plan.native = _native.NativePlan.create(plan)
+    return plan
+
+
+def plan_desc_fields(plan: AcquisitionPlan) -> dict:
+    """Scalar fields of `fmdbzp_plan_desc` for this plan."""
+    return dict(
+        samples_per_code=plan.samples_per_code,
+        num_blocks=plan.num_blocks,
+        block_size=plan.block_size,
+        fft_size=plan.fft_size,
+        num_coh=plan.num_coherently_integrated_code_periods,
+        num_data_bits=plan.num_data_bits,
+        bit_edge_search_steps=plan.bit_edge_search_steps,
+        num_noncoherent=plan.num_noncoherent_accumulations,
+        num_secondary_rotations=plan.num_secondary_rotations,
+        num_doppler_bins=plan.num_doppler_bins,
+        samples_per_code_eff=plan.samples_per_code_eff,
+        num_patterns=plan.num_patterns,
+        path=plan.path,
+        noise_estimator=plan.noise_estimator_id,
+        samples_per_chip=plan.samples_per_chip,
+        precision=0,
+        sampling_freq_hz=plan.sampling_freq,
+    )
+
+
+def plan_code_samples(plan: AcquisitionPlan) -> np.ndarray:
+    """nprn x spc int8: gen_code(spc, system, prn, fs, f_code, 0) per plan PRN
+    (src/plan.jl:241)."""
+    sysm = plan.system
+    return np.stack([sysm.gen_code(plan.samples_per_code, p, plan.sampling_freq, sysm.get_code_frequency(), 0.0)
+                     for p in plan.avail_prns]).astype(np.int8)
+
+
+def plan_patterns(plan: AcquisitionPlan):
+    """(npattern_sets, patterns (nsets, Q, N_coh) float32, pattern_set per PRN)."""
+    if plan.path == PATH_SIMPLE:
+        return 0, np.ones((1, 1, plan.num_coherently_integrated_code_periods), dtype=np.float32), \
+            np.zeros(len(plan.avail_prns), dtype=np.int32)
+    sets, index = [], []
+    ids = {}
+    for p in plan.avail_prns:
+        m = plan.sign_patterns_by_prn[p]
+        key = id(m)
+        if key not in ids:
+            ids[key] = len(sets)
+            sets.append(np.ascontiguousarray(m.T))  # (Q, N_coh): column-major (N_coh, Q)
+        index.append(ids[key])
+    return len(sets), np.stack(sets).astype(np.float32), np.array(index, dtype=np.int32)

@@ -1,0 +1,240 @@
+// fft_lds.hpp — workgroup-cooperative Stockham FFT in LDS for gfx950.
+//
+// Every FFT of the FM-DBZP search runs through here: the signal double-block
+// and code sub-block forward FFTs (size F), the block-correlation inverse FFTs
+// (size F) and the column Doppler FFTs (size ndop, batched over columns).
+// The reference computes the same transforms with FFTW
+// (src/coherent_integration.jl:51, :165; src/noncoherent_integration.jl:38,
+// :83, :224; src/plan.jl:247).
+//
+// Algorithm: Stockham autosort, one LDS pass per radix stage, ping-pong
+// between two LDS buffers, natural order in and out. Stage s of radix R with
+// Ns = product of earlier radices maps butterfly j (of n/R) to inputs
+// j + r n/R and outputs (j - j mod Ns) R + (j mod Ns) + r Ns, with twiddle
+// exp(-2 pi i (j mod Ns) r / (Ns R)) read from one FP64-computed table of n
+// entries. Radices 2, 3, 4, 5, 8 have unrolled butterflies; any other prime
+// uses a direct R-point DFT read straight from LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define FFT_MAX_STAGES 24
+
+struct FftPlanDev {
+    int n;
+    int nst;
+    int radix[FFT_MAX_STAGES];
+};
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 caddf(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csubf(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a)
+{
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ float2 twid(const float2* __restrict__ tw, int idx)
+{
+    float2 w = tw[idx];
+    if (INV) w.y = -w.y;
+    return w;
+}
+
+template <bool INV>
+__device__ __forceinline__ void bfly2(float2* v)
+{
+    float2 a = v[0], b = v[1];
+    v[0] = caddf(a, b);
+    v[1] = csubf(a, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void bfly4(float2* v)
+{
+    float2 a = caddf(v[0], v[2]);
+    float2 b = csubf(v[0], v[2]);
+    float2 c = caddf(v[1], v[3]);
+    float2 d = mul_mi<INV>(csubf(v[1], v[3]));
+    v[0] = caddf(a, c);
+    v[2] = csubf(a, c);
+    v[1] = caddf(b, d);
+    v[3] = csubf(b, d);
+}
+
+template <bool INV>
+__device__ __forceinline__ void bfly8(float2* v)
+{
+    const float h = 0.70710678118654752440f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        a[i] = caddf(v[i], v[i + 4]);
+        b[i] = csubf(v[i], v[i + 4]);
+    }
+    // b_i *= W8^i ; W8 = exp(-+ i pi/4)
+    {
+        float2 t = b[1];
+        // forward: (1 - i)/sqrt2 ; inverse: (1 + i)/sqrt2
+        b[1] = INV ? make_float2(h * (t.x - t.y), h * (t.x + t.y))
+                   : make_float2(h * (t.x + t.y), h * (t.y - t.x));
+        b[2] = mul_mi<INV>(b[2]);
+        t = b[3];
+        // forward: (-1 - i)/sqrt2 ; inverse: (-1 + i)/sqrt2
+        b[3] = INV ? make_float2(-h * (t.x + t.y), h * (t.x - t.y))
+                   : make_float2(h * (t.y - t.x), -h * (t.x + t.y));
+    }
+    bfly4<INV>(a);
+    bfly4<INV>(b);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[2 * i] = a[i];
+        v[2 * i + 1] = b[i];
+    }
+}
+
+template <bool INV>
+__device__ __forceinline__ void bfly3(float2* v)
+{
+    const float s = 0.86602540378443864676f;  // sin(2 pi / 3)
+    float2 t1 = caddf(v[1], v[2]);
+    float2 y0 = caddf(v[0], t1);
+    float2 t = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    float2 u = csubf(v[1], v[2]);
+    u = make_float2(s * u.x, s * u.y);
+    float2 mu = mul_mi<INV>(u);
+    v[0] = y0;
+    v[1] = caddf(t, mu);
+    v[2] = csubf(t, mu);
+}
+
+template <bool INV>
+__device__ __forceinline__ void bfly5(float2* v)
+{
+    const float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
+    const float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
+    const float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
+    const float s2 = 0.58778525229247312917f;   // sin(4 pi / 5)
+    float2 t1 = caddf(v[1], v[4]);
+    float2 t2 = caddf(v[2], v[3]);
+    float2 t3 = csubf(v[1], v[4]);
+    float2 t4 = csubf(v[2], v[3]);
+    float2 x0 = v[0];
+    float2 a1 = make_float2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+    float2 a2 = make_float2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+    float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    float2 mb1 = mul_mi<INV>(b1);
+    float2 mb2 = mul_mi<INV>(b2);
+    v[0] = caddf(x0, caddf(t1, t2));
+    v[1] = caddf(a1, mb1);
+    v[4] = csubf(a1, mb1);
+    v[2] = caddf(a2, mb2);
+    v[3] = csubf(a2, mb2);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void bfly(float2* v)
+{
+    if constexpr (R == 2) bfly2<INV>(v);
+    else if constexpr (R == 3) bfly3<INV>(v);
+    else if constexpr (R == 4) bfly4<INV>(v);
+    else if constexpr (R == 5) bfly5<INV>(v);
+    else bfly8<INV>(v);
+}
+
+// One Stockham stage with a compile-time radix over `nbatch` transforms of
+// length n laid out at stride `bstride` in LDS.
+template <int R, bool INV>
+__device__ __forceinline__ void stage_fixed(const float2* __restrict__ src, float2* __restrict__ dst,
+                                            int n, int Ns, int nbatch, int bstride,
+                                            const float2* __restrict__ tw, int tid, int nthr)
+{
+    const int m = n / R;
+    const int tstride = n / (Ns * R);
+    const int total = nbatch * m;
+    for (int idx = tid; idx < total; idx += nthr) {
+        const int t = idx / m;
+        const int j = idx - t * m;
+        const float2* in = src + t * bstride;
+        float2* out = dst + t * bstride;
+        const int k = j % Ns;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = in[j + r * m];
+        if (Ns > 1) {
+            const int ts = k * tstride;
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmulf(v[r], twid<INV>(tw, r * ts));
+        }
+        bfly<R, INV>(v);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) out[base + r * Ns] = v[r];
+    }
+}
+
+// Stage with an arbitrary prime radix: direct R-point DFT per output.
+template <bool INV>
+__device__ void stage_generic(const float2* __restrict__ src, float2* __restrict__ dst, int n, int R,
+                              int Ns, int nbatch, int bstride, const float2* __restrict__ tw,
+                              int tid, int nthr)
+{
+    const int m = n / R;
+    const int tstride = n / (Ns * R);
+    const int total = nbatch * m * R;
+    for (int idx = tid; idx < total; idx += nthr) {
+        const int t = idx / (m * R);
+        const int rem = idx - t * (m * R);
+        const int s = rem / m;       // output index within the butterfly
+        const int j = rem - s * m;   // butterfly
+        const float2* in = src + t * bstride;
+        float2* out = dst + t * bstride;
+        const int k = j % Ns;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int r = 0; r < R; r++) {
+            float2 v = in[j + r * m];
+            if (Ns > 1 && r > 0) v = cmulf(v, twid<INV>(tw, r * k * tstride));
+            // W_R^{r s} = tw[((r s) mod R) * (n / R)]
+            const int e = (r * s) % R;
+            if (e) v = cmulf(v, twid<INV>(tw, e * m));
+            acc = caddf(acc, v);
+        }
+        const int base = (j - k) * R + k;
+        out[base + s * Ns] = acc;
+    }
+}
+
+// Run the whole transform. Data starts in `a` (all threads must have written
+// it and passed a barrier); returns the buffer (a or b) holding the result,
+// after a final barrier.
+template <bool INV>
+__device__ float2* fft_lds(float2* a, float2* b, const FftPlanDev& p, const float2* __restrict__ tw,
+                           int nbatch, int bstride, int tid, int nthr)
+{
+    float2* src = a;
+    float2* dst = b;
+    int Ns = 1;
+    for (int s = 0; s < p.nst; s++) {
+        const int R = p.radix[s];
+        switch (R) {
+        case 2: stage_fixed<2, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
+        case 3: stage_fixed<3, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
+        case 4: stage_fixed<4, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
+        case 5: stage_fixed<5, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
+        case 8: stage_fixed<8, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
+        default: stage_generic<INV>(src, dst, p.n, R, Ns, nbatch, bstride, tw, tid, nthr); break;
+        }
+        __syncthreads();
+        float2* t = src;
+        src = dst;
+        dst = t;
+        Ns *= R;
+    }
+    return src;
+}

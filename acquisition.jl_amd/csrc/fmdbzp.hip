@@ -1,0 +1,661 @@
+// fmdbzp.hip — C ABI of the MI355X FM-DBZP acquisition engine.
+//
+// One device-resident plan per `plan_acquire` (src/plan.jl:312-697) and one
+// `fmdbzp_acquire` per `acquire!` (src/acquire.jl:54-215). A call runs, on
+// the plan's own HIP stream:
+//   1. k_signal_fft     downconvert + double-block FFTs for all N_nc segments
+//   2. per PRN batch, per step: k_corr_tile -> k_column_{simple,sign,rotation}
+//   3. per PRN batch: k_reduce_rows -> k_finalize (-> k_export when surfaces
+//      are requested)
+// and copies the per-PRN statistics back. PRN batches bound the workspace
+// (tile + accumulator) to a budget; the reference bounds its CPU footprint
+// the same way with a scratch slot per thread (src/plan.jl:574-608).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fmdbzp.h"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int set_err(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return set_err(e_ == hipErrorOutOfMemory ? FMDBZP_ENOMEM : FMDBZP_EHIP,       \
+                           "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),         \
+                           __FILE__, __LINE__);                                           \
+    } while (0)
+
+// exp(-2 pi i m / n) in FP64 with exact values at multiples of pi/2.
+void sincospi_d(double x, double* s, double* c)
+{
+    double r = x - 2.0 * nearbyint(x * 0.5);
+    double sgn_c = 1.0;
+    if (r > 0.5) { r = 1.0 - r; sgn_c = -1.0; }
+    else if (r < -0.5) { r = -1.0 - r; sgn_c = -1.0; }
+    double ar = fabs(r), ss, cc;
+    if (ar <= 0.25) { ss = sin(M_PI * ar); cc = cos(M_PI * ar); }
+    else { double t = 0.5 - ar; ss = cos(M_PI * t); cc = sin(M_PI * t); }
+    if (r < 0) ss = -ss;
+    *s = ss;
+    *c = sgn_c * cc;
+}
+
+std::vector<float2> twiddles(int n)
+{
+    std::vector<float2> tw((size_t)n);
+    for (int m = 0; m < n; m++) {
+        double s, c;
+        sincospi_d(2.0 * (double)m / (double)n, &s, &c);
+        tw[(size_t)m] = make_float2((float)c, (float)(-s));
+    }
+    return tw;
+}
+
+// Radix schedule: 8s, then one 4 or 2, then 5s, 3s, 7s and larger primes.
+int make_fft_plan(int n, FftPlanDev* p)
+{
+    p->n = n;
+    p->nst = 0;
+    int m = n;
+    auto push = [&](int r) {
+        if (p->nst >= FFT_MAX_STAGES) return false;
+        p->radix[p->nst++] = r;
+        return true;
+    };
+    while (m % 8 == 0) { if (!push(8)) return -1; m /= 8; }
+    while (m % 4 == 0) { if (!push(4)) return -1; m /= 4; }
+    while (m % 2 == 0) { if (!push(2)) return -1; m /= 2; }
+    while (m % 5 == 0) { if (!push(5)) return -1; m /= 5; }
+    while (m % 3 == 0) { if (!push(3)) return -1; m /= 3; }
+    for (int f = 7; m > 1; f += 2)
+        while (m % f == 0) { if (!push(f)) return -1; m /= f; }
+    return 0;
+}
+
+constexpr size_t kLdsMax = 160 * 1024;   // gfx950 LDS per workgroup
+constexpr size_t kLdsDefault = 64 * 1024;
+
+struct TimedEvent {
+    int cls;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct fmdbzp_plan {
+    std::mutex mu;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    fmdbzp_plan_desc d{};
+    int nprn = 0;
+    int nsets = 0;
+    FftPlanDev pf{}, pd{};
+    float2* d_twF = nullptr;
+    float2* d_twD = nullptr;
+    float2* d_prnconj = nullptr;
+    float* d_patterns = nullptr;
+    int32_t* d_pset = nullptr;
+    int32_t* d_shifts = nullptr;
+    // workspace
+    void* d_signal = nullptr;
+    size_t signal_cap = 0;
+    float2* d_sigfft = nullptr;
+    float2* d_tile = nullptr;
+    float* d_acc = nullptr;
+    double* d_rowsum = nullptr;
+    float* d_rowbest_v = nullptr;
+    uint64_t* d_rowbest_k = nullptr;
+    int32_t* d_req = nullptr;
+    size_t req_cap = 0;
+    fmdbzp_prn_stats* d_stats = nullptr;
+    size_t stats_cap = 0;
+    float* d_export = nullptr;
+    int batch = 1;
+    int LB = 1, bstride = 1;
+    size_t col_lds = 0;
+    size_t fft_lds = 0;
+    float inv_f = 0.f;
+    // timing
+    bool timing = false;
+    std::vector<TimedEvent> pending;
+    std::vector<hipEvent_t> pool;
+    double tot_ms[FMDBZP_K_NUM_CLASSES] = {};
+    int64_t launches[FMDBZP_K_NUM_CLASSES] = {};
+};
+
+namespace {
+
+size_t elsize(int eltype)
+{
+    switch (eltype) {
+    case FMDBZP_CF32: return 8;
+    case FMDBZP_CF64: return 16;
+    case FMDBZP_I16: return 2;
+    case FMDBZP_CI8: return 2;
+    case FMDBZP_CI16: return 4;
+    case FMDBZP_F32: return 4;
+    case FMDBZP_F64: return 8;
+    default: return 0;
+    }
+}
+
+hipEvent_t get_event(fmdbzp_plan* P)
+{
+    if (!P->pool.empty()) {
+        hipEvent_t e = P->pool.back();
+        P->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch bracket: records HIP events on the plan's stream when timing is on.
+struct Timed {
+    fmdbzp_plan* P;
+    TimedEvent ev{};
+    bool on;
+    Timed(fmdbzp_plan* p, int cls) : P(p), on(p->timing)
+    {
+        if (on) {
+            ev.cls = cls;
+            ev.start = get_event(P);
+            ev.stop = get_event(P);
+            if (!ev.start || !ev.stop) on = false;
+            else hipEventRecord(ev.start, P->stream);
+        }
+    }
+    ~Timed()
+    {
+        if (on) {
+            hipEventRecord(ev.stop, P->stream);
+            P->pending.push_back(ev);
+        }
+    }
+};
+
+void collect_timing(fmdbzp_plan* P)
+{
+    for (auto& e : P->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.start, e.stop) == hipSuccess) {
+            P->tot_ms[e.cls] += ms;
+            P->launches[e.cls] += 1;
+        }
+        P->pool.push_back(e.start);
+        P->pool.push_back(e.stop);
+    }
+    P->pending.clear();
+}
+
+int validate_desc(const fmdbzp_plan_desc* d)
+{
+    if (d->abi_version != FMDBZP_ABI_VERSION)
+        return set_err(FMDBZP_EINVAL, "abi_version %d != %d", d->abi_version, FMDBZP_ABI_VERSION);
+    if (d->samples_per_code <= 0 || d->num_blocks <= 0 || d->block_size <= 0 || d->fft_size <= 0 ||
+        d->num_coh <= 0 || d->num_data_bits <= 0 || d->bit_edge_search_steps <= 0 ||
+        d->num_noncoherent <= 0 || d->num_secondary_rotations <= 0 || d->num_doppler_bins <= 0 ||
+        d->num_patterns <= 0)
+        return set_err(FMDBZP_EINVAL, "descriptor has a non-positive size");
+    if ((int64_t)d->num_blocks * d->block_size != d->samples_per_code)
+        return set_err(FMDBZP_EINVAL, "num_blocks * block_size != samples_per_code");
+    if ((int64_t)d->num_coh * d->num_blocks != d->num_doppler_bins)
+        return set_err(FMDBZP_EINVAL, "num_doppler_bins != num_coh * num_blocks");
+    if (d->fft_size < 2 * d->block_size)
+        return set_err(FMDBZP_EINVAL, "fft_size %d < 2 * block_size %d", d->fft_size, d->block_size);
+    if ((int64_t)d->samples_per_code * d->num_secondary_rotations != d->samples_per_code_eff)
+        return set_err(FMDBZP_EINVAL, "samples_per_code_eff != samples_per_code * rotations");
+    if (d->num_coh % d->num_data_bits != 0)
+        return set_err(FMDBZP_EINVAL, "num_coh %% num_data_bits != 0");
+    if (d->num_data_bits > 30) return set_err(FMDBZP_EINVAL, "num_data_bits too large");
+    if (d->path < 0 || d->path > 2) return set_err(FMDBZP_EINVAL, "bad path %d", d->path);
+    if (d->path == FMDBZP_PATH_ROTATION &&
+        d->num_patterns != (1 << (d->num_data_bits - 1)) * d->num_secondary_rotations)
+        return set_err(FMDBZP_EINVAL, "rotation path needs Q = 2^(N_db-1) * R patterns");
+    if (d->path == FMDBZP_PATH_SIGN && d->num_patterns != (1 << (d->num_data_bits - 1)))
+        return set_err(FMDBZP_EINVAL, "sign path needs Q = 2^(N_db-1) patterns");
+    if (d->path != FMDBZP_PATH_ROTATION && d->num_secondary_rotations != 1)
+        return set_err(FMDBZP_EINVAL, "rotations > 1 only on the rotation path");
+    if (d->noise_estimator != FMDBZP_NOISE_OPPOSITE_ROW && d->noise_estimator != FMDBZP_NOISE_GLOBAL_MEAN)
+        return set_err(FMDBZP_EINVAL, "bad noise estimator %d", d->noise_estimator);
+    if (d->precision != FMDBZP_PREC_FP32) return set_err(FMDBZP_EINVAL, "unsupported precision %d", d->precision);
+    if (d->sampling_freq_hz <= 0) return set_err(FMDBZP_EINVAL, "sampling_freq_hz <= 0");
+    // LDS: the F-point transforms need two F-entry buffers in one workgroup
+    if ((size_t)d->fft_size * 2 * sizeof(float2) > kLdsMax)
+        return set_err(FMDBZP_EINVAL, "fft_size %d exceeds the LDS transform limit", d->fft_size);
+    // the column stage needs three ndop-entry complex buffers + one float buffer
+    if ((size_t)(d->num_doppler_bins | 1) * 28 > kLdsMax)
+        return set_err(FMDBZP_EINVAL, "num_doppler_bins %d exceeds the LDS column limit", d->num_doppler_bins);
+    int64_t seglen = (int64_t)d->num_coh * d->samples_per_code;
+    if (seglen * d->num_noncoherent > ((int64_t)1 << 40)) return set_err(FMDBZP_EINVAL, "signal too long");
+    return FMDBZP_OK;
+}
+
+int set_lds_attr(const void* func, size_t bytes)
+{
+    if (bytes > kLdsDefault) {
+        hipError_t e = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess)
+            return set_err(FMDBZP_EHIP, "hipFuncSetAttribute(%zu bytes LDS): %s", bytes, hipGetErrorString(e));
+    }
+    return FMDBZP_OK;
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count)
+{
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess)
+        return set_err(e == hipErrorOutOfMemory ? FMDBZP_ENOMEM : FMDBZP_EHIP, "hipMalloc(%zu bytes): %s",
+                       count * sizeof(T), hipGetErrorString(e));
+    return FMDBZP_OK;
+}
+
+void free_plan(fmdbzp_plan* P)
+{
+    if (!P) return;
+    hipSetDevice(P->device);
+    if (P->stream) hipStreamSynchronize(P->stream);
+    hipFree(P->d_twF);
+    hipFree(P->d_twD);
+    hipFree(P->d_prnconj);
+    hipFree(P->d_patterns);
+    hipFree(P->d_pset);
+    hipFree(P->d_shifts);
+    hipFree(P->d_signal);
+    hipFree(P->d_sigfft);
+    hipFree(P->d_tile);
+    hipFree(P->d_acc);
+    hipFree(P->d_rowsum);
+    hipFree(P->d_rowbest_v);
+    hipFree(P->d_rowbest_k);
+    hipFree(P->d_req);
+    hipFree(P->d_stats);
+    hipFree(P->d_export);
+    for (auto& e : P->pending) { hipEventDestroy(e.start); hipEventDestroy(e.stop); }
+    for (auto e : P->pool) hipEventDestroy(e);
+    if (P->stream) hipStreamDestroy(P->stream);
+    delete P;
+}
+
+// Workspace budget for the per-batch tile + accumulator, bytes.
+size_t workspace_budget()
+{
+    const char* s = getenv("FMDBZP_WORKSPACE_GB");
+    double gb = s ? atof(s) : 32.0;
+    if (!(gb > 0)) gb = 32.0;
+    size_t free_b = 0, total_b = 0;
+    size_t want = (size_t)(gb * 1024.0 * 1024.0 * 1024.0);
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) {
+        size_t cap = free_b / 2;
+        if (want > cap) want = cap;
+    }
+    return want;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fmdbzp_last_error(void) { return g_err.c_str(); }
+
+int fmdbzp_query(char* buf, size_t buflen)
+{
+    if (!buf || buflen == 0) return set_err(FMDBZP_EINVAL, "null buffer");
+    int dev = 0, n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        snprintf(buf, buflen, "abi=%d devices=0", FMDBZP_ABI_VERSION);
+        return set_err(FMDBZP_ENODEV, "no HIP device: %s", hipGetErrorString(e));
+    }
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    snprintf(buf, buflen, "abi=%d device=%d name=%s arch=%s cus=%d lds_per_block=%zu hbm_bytes=%zu",
+             FMDBZP_ABI_VERSION, dev, prop.name, prop.gcnArchName, prop.multiProcessorCount,
+             (size_t)prop.sharedMemPerBlock, (size_t)prop.totalGlobalMem);
+    return FMDBZP_OK;
+}
+
+int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t* code_samples,
+                       int32_t npattern_sets, const float* patterns, const int32_t* pattern_set,
+                       const int32_t* drift_shifts, fmdbzp_plan** out_plan)
+{
+    g_err.clear();
+    if (!desc || !out_plan || !code_samples || !drift_shifts)
+        return set_err(FMDBZP_EINVAL, "null argument to fmdbzp_plan_create");
+    *out_plan = nullptr;
+    int rc = validate_desc(desc);
+    if (rc) return rc;
+    if (nprn <= 0) return set_err(FMDBZP_EINVAL, "nprn must be positive");
+    const fmdbzp_plan_desc& d = *desc;
+    if (d.path != FMDBZP_PATH_SIMPLE) {
+        if (npattern_sets <= 0 || !patterns)
+            return set_err(FMDBZP_EINVAL, "sign/rotation path needs sign patterns");
+        if (npattern_sets > 1 && !pattern_set) return set_err(FMDBZP_EINVAL, "pattern_set is NULL");
+    }
+    for (int64_t i = 0; i < (int64_t)d.num_noncoherent * d.num_doppler_bins; i++)
+        if (drift_shifts[i] < 0 || drift_shifts[i] >= d.samples_per_code)
+            return set_err(FMDBZP_EINVAL, "drift shift %d out of [0, spc)", drift_shifts[i]);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(FMDBZP_ENODEV, "no HIP device visible");
+
+    fmdbzp_plan* P = new fmdbzp_plan();
+    P->d = d;
+    P->nprn = nprn;
+    P->nsets = d.path == FMDBZP_PATH_SIMPLE ? 0 : npattern_sets;
+    hipGetDevice(&P->device);
+    auto fail = [&](int code) {
+        free_plan(P);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(FMDBZP_EHIP, "hipStreamCreate failed"));
+    if (make_fft_plan(d.fft_size, &P->pf) || make_fft_plan(d.num_doppler_bins, &P->pd))
+        return fail(set_err(FMDBZP_EINVAL, "FFT size has too many factors"));
+    const int F = d.fft_size, ndop = d.num_doppler_bins, nb = d.num_blocks, bs = d.block_size;
+    const int spc = d.samples_per_code, spc_eff = d.samples_per_code_eff;
+    P->inv_f = 1.0f / (float)F;
+    P->fft_lds = (size_t)2 * F * sizeof(float2);
+
+    // column-stage geometry: LB columns per workgroup, padded column stride
+    P->bstride = ndop | 1;
+    const size_t per_col = d.path == FMDBZP_PATH_SIMPLE ? (size_t)P->bstride * 2 * sizeof(float2)
+                                                         : (size_t)P->bstride * 3 * sizeof(float2) +
+                                                               (size_t)ndop * sizeof(float);
+    int LB = (int)(kLdsDefault / per_col);
+    if (LB > 64) LB = 64;
+    if (LB < 1) LB = 1;
+    if (LB > bs) LB = bs;
+    P->LB = LB;
+    P->col_lds = per_col * LB;
+    if (P->col_lds > kLdsMax) return fail(set_err(FMDBZP_EINVAL, "column stage does not fit LDS"));
+
+    if ((rc = set_lds_attr((const void*)k_code_fft, P->fft_lds)) ||
+        (rc = set_lds_attr((const void*)k_signal_fft, P->fft_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile, P->fft_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_simple, P->col_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_sign, P->col_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_rotation, P->col_lds)))
+        return fail(rc);
+
+    // tables
+    {
+        std::vector<float2> twF = twiddles(F), twD = twiddles(ndop);
+        if ((rc = dev_alloc(&P->d_twF, (size_t)F)) || (rc = dev_alloc(&P->d_twD, (size_t)ndop)))
+            return fail(rc);
+        if (hipMemcpy(P->d_twF, twF.data(), sizeof(float2) * F, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(P->d_twD, twD.data(), sizeof(float2) * ndop, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(FMDBZP_EHIP, "twiddle upload failed"));
+    }
+    if ((rc = dev_alloc(&P->d_prnconj, (size_t)nprn * nb * F))) return fail(rc);
+    if ((rc = dev_alloc(&P->d_shifts, (size_t)d.num_noncoherent * ndop))) return fail(rc);
+    if (hipMemcpy(P->d_shifts, drift_shifts, sizeof(int32_t) * (size_t)d.num_noncoherent * ndop,
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(FMDBZP_EHIP, "drift shift upload failed"));
+    {
+        int nsets = P->nsets > 0 ? P->nsets : 1;
+        size_t npat = (size_t)nsets * d.num_coh * d.num_patterns;
+        if ((rc = dev_alloc(&P->d_patterns, npat)) || (rc = dev_alloc(&P->d_pset, (size_t)nprn))) return fail(rc);
+        std::vector<int32_t> pset((size_t)nprn, 0);
+        if (P->nsets > 1) {
+            for (int i = 0; i < nprn; i++) {
+                if (pattern_set[i] < 0 || pattern_set[i] >= P->nsets)
+                    return fail(set_err(FMDBZP_EINVAL, "pattern_set[%d] = %d out of range", i, pattern_set[i]));
+                pset[(size_t)i] = pattern_set[i];
+            }
+        }
+        if (hipMemcpy(P->d_pset, pset.data(), sizeof(int32_t) * nprn, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(FMDBZP_EHIP, "pattern set upload failed"));
+        if (P->nsets > 0) {
+            for (size_t i = 0; i < npat; i++)
+                if (patterns[i] != 1.0f && patterns[i] != -1.0f)
+                    return fail(set_err(FMDBZP_EINVAL, "sign patterns must be +1/-1"));
+            if (hipMemcpy(P->d_patterns, patterns, sizeof(float) * npat, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(set_err(FMDBZP_EHIP, "pattern upload failed"));
+        }
+    }
+    // code samples -> conj sub-block FFTs on the device
+    {
+        for (int64_t i = 0; i < (int64_t)nprn * spc; i++)
+            if (code_samples[i] != 1 && code_samples[i] != -1)
+                return fail(set_err(FMDBZP_EINVAL, "code samples must be +1/-1"));
+        int8_t* d_codes = nullptr;
+        if ((rc = dev_alloc(&d_codes, (size_t)nprn * spc))) return fail(rc);
+        hipError_t e = hipMemcpy(d_codes, code_samples, (size_t)nprn * spc, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            Timed t(P, FMDBZP_K_CODE_FFT);
+            k_code_fft<<<dim3(nb, nprn), FMDBZP_NTHR, P->fft_lds, P->stream>>>(d_codes, P->d_prnconj, P->pf,
+                                                                              P->d_twF, spc, nb, bs);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+        hipFree(d_codes);
+        if (e != hipSuccess) return fail(set_err(FMDBZP_EHIP, "code FFT failed: %s", hipGetErrorString(e)));
+        collect_timing(P);
+    }
+    // workspace: signal FFT cache for all segments + PRN-batch tile/accumulator
+    if ((rc = dev_alloc(&P->d_sigfft, (size_t)d.num_noncoherent * ndop * F))) return fail(rc);
+    {
+        const size_t per_prn = (size_t)nb * ndop * bs * sizeof(float2) + (size_t)ndop * spc_eff * sizeof(float);
+        size_t budget = workspace_budget();
+        size_t b = budget / per_prn;
+        if (b < 1) b = 1;
+        if (b > (size_t)nprn) b = (size_t)nprn;
+        if (b > 65535) b = 65535;
+        P->batch = (int)b;
+    }
+    if ((rc = dev_alloc(&P->d_tile, (size_t)P->batch * nb * ndop * bs)) ||
+        (rc = dev_alloc(&P->d_acc, (size_t)P->batch * ndop * spc_eff)) ||
+        (rc = dev_alloc(&P->d_rowsum, (size_t)P->batch * ndop)) ||
+        (rc = dev_alloc(&P->d_rowbest_v, (size_t)P->batch * ndop)) ||
+        (rc = dev_alloc(&P->d_rowbest_k, (size_t)P->batch * ndop)))
+        return fail(rc);
+    *out_plan = P;
+    return FMDBZP_OK;
+}
+
+int fmdbzp_plan_destroy(fmdbzp_plan* plan)
+{
+    free_plan(plan);
+    return FMDBZP_OK;
+}
+
+int fmdbzp_plan_set_timing(fmdbzp_plan* plan, int32_t enable)
+{
+    if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
+    std::lock_guard<std::mutex> lk(plan->mu);
+    plan->timing = enable != 0;
+    for (int i = 0; i < FMDBZP_K_NUM_CLASSES; i++) { plan->tot_ms[i] = 0; plan->launches[i] = 0; }
+    return FMDBZP_OK;
+}
+
+int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launches, int32_t n)
+{
+    if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
+    std::lock_guard<std::mutex> lk(plan->mu);
+    for (int i = 0; i < n && i < FMDBZP_K_NUM_CLASSES; i++) {
+        if (total_ms) total_ms[i] = plan->tot_ms[i];
+        if (launches) launches[i] = plan->launches[i];
+    }
+    return FMDBZP_K_NUM_CLASSES;
+}
+
+int fmdbzp_acquire(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_t n_samples,
+                   double interm_freq_hz, const int32_t* prn_index, int32_t nreq, uint32_t flags,
+                   fmdbzp_prn_stats* out_stats, float* const* out_surfaces)
+{
+    g_err.clear();
+    if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
+    fmdbzp_plan* P = plan;
+    std::lock_guard<std::mutex> lk(P->mu);
+    const fmdbzp_plan_desc& d = P->d;
+    if (!signal || !prn_index || !out_stats) return set_err(FMDBZP_EINVAL, "null argument to fmdbzp_acquire");
+    if (nreq <= 0) return set_err(FMDBZP_EINVAL, "nreq must be positive");
+    const size_t es = elsize(eltype);
+    if (!es) return set_err(FMDBZP_EINVAL, "unknown eltype %d", eltype);
+    const bool store = (flags & FMDBZP_FLAG_STORE_SURFACES) != 0;
+    if (store && !out_surfaces) return set_err(FMDBZP_EINVAL, "STORE_SURFACES without surface pointers");
+    for (int i = 0; i < nreq; i++)
+        if (prn_index[i] < 0 || prn_index[i] >= P->nprn)
+            return set_err(FMDBZP_EINVAL, "prn_index[%d] = %d out of range [0, %d)", i, prn_index[i], P->nprn);
+    const int F = d.fft_size, ndop = d.num_doppler_bins, nb = d.num_blocks, bs = d.block_size;
+    const int spc = d.samples_per_code, spc_eff = d.samples_per_code_eff, nnc = d.num_noncoherent;
+    const int64_t seglen = (int64_t)d.num_coh * spc;
+    if (n_samples / seglen < nnc)
+        return set_err(FMDBZP_EINVAL, "signal has %lld samples, fewer than %d segments of %lld",
+                       (long long)n_samples, nnc, (long long)seglen);
+    HIP_TRY(hipSetDevice(P->device));
+
+    // signal: only the N_nc segments the search reads cross the bus
+    const size_t sig_bytes = (size_t)seglen * nnc * es;
+    const void* d_sig = signal;
+    if (!(flags & FMDBZP_FLAG_DEVICE_SIGNAL)) {
+        if (P->signal_cap < sig_bytes) {
+            hipFree(P->d_signal);
+            P->d_signal = nullptr;
+            P->signal_cap = 0;
+            HIP_TRY(hipMalloc(&P->d_signal, sig_bytes));
+            P->signal_cap = sig_bytes;
+        }
+        HIP_TRY(hipMemcpyAsync(P->d_signal, signal, sig_bytes, hipMemcpyHostToDevice, P->stream));
+        d_sig = P->d_signal;
+    }
+    if (P->req_cap < (size_t)nreq) {
+        hipFree(P->d_req);
+        hipFree(P->d_stats);
+        P->d_req = nullptr;
+        P->d_stats = nullptr;
+        P->req_cap = 0;
+        HIP_TRY(hipMalloc((void**)&P->d_req, sizeof(int32_t) * nreq));
+        HIP_TRY(hipMalloc((void**)&P->d_stats, sizeof(fmdbzp_prn_stats) * nreq));
+        P->req_cap = (size_t)nreq;
+    }
+    HIP_TRY(hipMemcpyAsync(P->d_req, prn_index, sizeof(int32_t) * nreq, hipMemcpyHostToDevice, P->stream));
+    if (store && !P->d_export && !(flags & FMDBZP_FLAG_DEVICE_OUTPUT))
+        HIP_TRY(hipMalloc((void**)&P->d_export, sizeof(float) * (size_t)ndop * spc_eff));
+
+    // 1. downconvert + double-block FFTs of every segment
+    {
+        const int do_if = interm_freq_hz != 0.0;
+        const double twopi = 6.283185307179586;  // Julia `2π`
+        const float step = (float)((-twopi * interm_freq_hz) / d.sampling_freq_hz);
+        Timed t(P, FMDBZP_K_SIGNAL_FFT);
+        k_signal_fft<<<dim3(ndop, nnc), FMDBZP_NTHR, P->fft_lds, P->stream>>>(
+            d_sig, eltype, P->d_sigfft, P->pf, P->d_twF, bs, seglen, ndop, do_if, step);
+        HIP_TRY(hipGetLastError());
+    }
+
+    const int ndb = d.num_data_bits;
+    const int ndc = 1 << (ndb - 1);
+    const int bpc = d.num_coh / ndb;
+    ColumnArgs A{};
+    A.patterns = P->d_patterns;
+    A.pset = P->d_pset;
+    A.twD = P->d_twD;
+    A.pd = P->pd;
+    A.nb = nb; A.bs = bs; A.ndop = ndop; A.spc = spc; A.spc_eff = spc_eff;
+    A.ncoh = d.num_coh; A.ndb = ndb; A.E = d.bit_edge_search_steps; A.R = d.num_secondary_rotations;
+    A.Q = d.num_patterns; A.ndc = ndc; A.edge_step = (bpc * nb) / d.bit_edge_search_steps;
+    A.use_max = (ndc > 1 || d.bit_edge_search_steps > 1) ? 1 : 0;
+    A.LB = P->LB;
+    A.bstride = P->bstride;
+    const int streamed_rot = (nnc == 1 && d.path == FMDBZP_PATH_ROTATION) ? 1 : 0;
+    const int nlb = (bs + P->LB - 1) / P->LB;
+
+    for (int b0 = 0; b0 < nreq; b0 += P->batch) {
+        const int nbat = nreq - b0 < P->batch ? nreq - b0 : P->batch;
+        const int32_t* req = P->d_req + b0;
+        A.tile = P->d_tile;
+        A.acc = P->d_acc;
+        A.req = req;
+        for (int s = 0; s < nnc; s++) {
+            {
+                Timed t(P, FMDBZP_K_CORR_TILE);
+                k_corr_tile<<<dim3(ndop, nb, nbat), FMDBZP_NTHR, P->fft_lds, P->stream>>>(
+                    P->d_sigfft + (size_t)s * ndop * F, P->d_prnconj, req, P->d_tile, P->pf, P->d_twF, nb, bs, ndop,
+                    P->inv_f);
+                HIP_TRY(hipGetLastError());
+            }
+            A.shifts = P->d_shifts + (size_t)s * ndop;
+            A.first_step = s == 0;
+            {
+                Timed t(P, FMDBZP_K_COLUMN);
+                dim3 g(nlb, nb, nbat);
+                if (d.path == FMDBZP_PATH_SIMPLE)
+                    k_column_simple<<<g, FMDBZP_NTHR, P->col_lds, P->stream>>>(A);
+                else if (d.path == FMDBZP_PATH_SIGN)
+                    k_column_sign<<<g, FMDBZP_NTHR, P->col_lds, P->stream>>>(A);
+                else
+                    k_column_rotation<<<g, FMDBZP_NTHR, P->col_lds, P->stream>>>(A);
+                HIP_TRY(hipGetLastError());
+            }
+        }
+        {
+            Timed t(P, FMDBZP_K_REDUCE);
+            k_reduce_rows<<<dim3(ndop, nbat), FMDBZP_NTHR, 0, P->stream>>>(
+                P->d_acc, ndop, spc, spc_eff, d.num_secondary_rotations, streamed_rot, P->d_rowsum, P->d_rowbest_v,
+                P->d_rowbest_k);
+            HIP_TRY(hipGetLastError());
+        }
+        {
+            Timed t(P, FMDBZP_K_FINALIZE);
+            k_finalize<<<dim3(nbat), FMDBZP_NTHR, 0, P->stream>>>(
+                P->d_acc, P->d_rowsum, P->d_rowbest_v, P->d_rowbest_k, ndop, spc, spc_eff, d.num_secondary_rotations,
+                streamed_rot, d.noise_estimator, d.samples_per_chip, P->d_stats + b0);
+            HIP_TRY(hipGetLastError());
+        }
+        if (store) {
+            for (int p = 0; p < nbat; p++) {
+                float* dst = out_surfaces[b0 + p];
+                if (!dst) continue;
+                const bool dev_out = (flags & FMDBZP_FLAG_DEVICE_OUTPUT) != 0;
+                float* target = dev_out ? dst : P->d_export;
+                {
+                    Timed t(P, FMDBZP_K_EXPORT);
+                    dim3 g((spc_eff + 31) / 32, (ndop + 31) / 32);
+                    k_export<<<g, dim3(32, 8), 0, P->stream>>>(P->d_acc + (size_t)p * ndop * spc_eff, target, ndop,
+                                                                spc_eff);
+                    HIP_TRY(hipGetLastError());
+                }
+                if (!dev_out)
+                    HIP_TRY(hipMemcpyAsync(dst, P->d_export, sizeof(float) * (size_t)ndop * spc_eff,
+                                           hipMemcpyDeviceToHost, P->stream));
+            }
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(out_stats, P->d_stats, sizeof(fmdbzp_prn_stats) * nreq, hipMemcpyDeviceToHost,
+                           P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    if (P->timing) collect_timing(P);
+    return FMDBZP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,550 @@
+// kernels.hpp — FM-DBZP search kernels for gfx950 (MI355X).
+//
+// Data layout in HBM (all complex values are interleaved float2):
+//   prnconj [prn][b][F]          conj FFT of code sub-block b      (plan.jl:230-251)
+//   sigfft  [step][k][F]         FFT of signal double block k      (coherent_integration.jl:12-56)
+//   tile    [p][r][k][lag]       block correlation, lag < bs       (coherent_integration.jl:136-171)
+//   acc     [p][row][col_eff]    power surface, sorted-Doppler rows, scrambled FM-DBZP
+//                                columns, accumulated over the N_nc steps
+//                                (noncoherent_integration.jl:760-812)
+// p indexes the PRNs of the current batch; the PRN is req[p].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fft_lds.hpp"
+#include "../../include/fmdbzp.h"
+
+#define FMDBZP_NTHR 256
+
+// ComplexF32(x) of one input sample (src/acquire.jl:94, :101).
+__device__ __forceinline__ float2 load_sample(const void* __restrict__ sig, int eltype, int64_t i)
+{
+    switch (eltype) {
+    case FMDBZP_CF32: return ((const float2*)sig)[i];
+    case FMDBZP_CF64: {
+        const double2 v = ((const double2*)sig)[i];
+        return make_float2((float)v.x, (float)v.y);
+    }
+    case FMDBZP_I16: return make_float2((float)((const int16_t*)sig)[i], 0.f);
+    case FMDBZP_CI8: {
+        const char2 v = ((const char2*)sig)[i];
+        return make_float2((float)v.x, (float)v.y);
+    }
+    case FMDBZP_CI16: {
+        const short2 v = ((const short2*)sig)[i];
+        return make_float2((float)v.x, (float)v.y);
+    }
+    case FMDBZP_F32: return make_float2(((const float*)sig)[i], 0.f);
+    case FMDBZP_F64: return make_float2((float)((const double*)sig)[i], 0.f);
+    default: return make_float2(0.f, 0.f);
+    }
+}
+
+// |z|^2 as Julia's abs2: re*re + im*im, never fused.
+__device__ __forceinline__ float abs2_exact(float2 z)
+{
+#pragma clang fp contract(off)
+    return z.x * z.x + z.y * z.y;
+}
+
+// ---------------------------------------------------------------------------
+// Plan time: conjugated code sub-block FFTs (src/plan.jl:230-251).
+// grid (nb, nprn)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_code_fft(const int8_t* __restrict__ codes, float2* __restrict__ prnconj, FftPlanDev pf,
+           const float2* __restrict__ twF, int spc, int nb, int bs)
+{
+    extern __shared__ float2 lds[];
+    const int F = pf.n;
+    float2* a = lds;
+    float2* b = lds + F;
+    const int blk = blockIdx.x, prn = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int8_t* code = codes + (size_t)prn * spc + (size_t)blk * bs;
+    for (int i = tid; i < F; i += FMDBZP_NTHR) a[i] = make_float2(i < bs ? (float)code[i] : 0.f, 0.f);
+    __syncthreads();
+    float2* res = fft_lds<false>(a, b, pf, twF, 1, F, tid, FMDBZP_NTHR);
+    float2* out = prnconj + ((size_t)prn * nb + blk) * F;
+    for (int i = tid; i < F; i += FMDBZP_NTHR) out[i] = make_float2(res[i].x, -res[i].y);
+}
+
+// ---------------------------------------------------------------------------
+// Downconversion fused into the signal double-block FFTs.
+// `_downconvert!` (src/acquire.jl:92-104) + `_precompute_signal_block_ffts!`
+// (src/coherent_integration.jl:12-56). The IF phase argument is formed in
+// Float32 with separately rounded multiply and add, as the reference does;
+// sin/cos of that Float32 argument are evaluated in FP64 and rounded.
+// grid (ncoh*nb, nnc)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_signal_fft(const void* __restrict__ sig, int eltype, float2* __restrict__ sigfft, FftPlanDev pf,
+             const float2* __restrict__ twF, int bs, int64_t seglen, int ndop, int do_if,
+             float phase_step)
+{
+    extern __shared__ float2 lds[];
+    const int F = pf.n;
+    float2* a = lds;
+    float2* b = lds + F;
+    const int gk = blockIdx.x, step = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int64_t seg0 = (int64_t)step * seglen;
+    const float off = __fmul_rn((float)seg0, phase_step);  // Float32(seg_start-1) * step
+    for (int i = tid; i < F; i += FMDBZP_NTHR) {
+        float2 v = make_float2(0.f, 0.f);
+        if (i < 2 * bs) {
+            int64_t j = (int64_t)gk * bs + i;
+            if (j >= seglen) j -= seglen;  // last double block wraps (coherent_integration.jl:44-50)
+            v = load_sample(sig, eltype, seg0 + j);
+            if (do_if) {
+#pragma clang fp contract(off)
+                const float ph = off + (float)j * phase_step;
+                const double dph = (double)ph;
+                const float s = (float)sin(dph), c = (float)cos(dph);
+                v = make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+            }
+        }
+        a[i] = v;
+    }
+    __syncthreads();
+    float2* res = fft_lds<false>(a, b, pf, twF, 1, F, tid, FMDBZP_NTHR);
+    float2* out = sigfft + ((size_t)step * ndop + gk) * F;
+    for (int i = tid; i < F; i += FMDBZP_NTHR) out[i] = res[i];
+}
+
+// ---------------------------------------------------------------------------
+// Block correlation: `_build_coherent_tile!` (src/coherent_integration.jl:136-171).
+// Row k of column block r correlates signal double block k with code sub-block
+// (k mod nb + r) mod nb: pointwise product in the frequency domain, inverse
+// FFT (unnormalised), times Float32(1/F), first bs lags kept.
+// grid (ndop, nb, batch)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
+            const int32_t* __restrict__ req, float2* __restrict__ tile, FftPlanDev pf,
+            const float2* __restrict__ twF, int nb, int bs, int ndop, float inv_f)
+{
+    extern __shared__ float2 lds[];
+    const int F = pf.n;
+    float2* a = lds;
+    float2* b = lds + F;
+    const int k = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int tid = threadIdx.x;
+    const int prn = req[p];
+    const int pb = (k % nb + r) % nb;
+    const float2* S = sigfft_step + (size_t)k * F;
+    const float2* P = prnconj + ((size_t)prn * nb + pb) * F;
+    for (int i = tid; i < F; i += FMDBZP_NTHR) a[i] = cmulf(S[i], P[i]);
+    __syncthreads();
+    float2* res = fft_lds<true>(a, b, pf, twF, 1, F, tid, FMDBZP_NTHR);
+    float2* out = tile + (((size_t)p * nb + r) * ndop + k) * bs;
+    for (int i = tid; i < bs; i += FMDBZP_NTHR) out[i] = make_float2(res[i].x * inv_f, res[i].y * inv_f);
+}
+
+// ---------------------------------------------------------------------------
+// Column stage. One workgroup owns LB consecutive lags (FM-DBZP columns) of
+// one (PRN, column block) tile, with all ndop rows of each column in LDS.
+// ---------------------------------------------------------------------------
+struct ColumnArgs {
+    const float2* tile;      // [batch][nb][ndop][bs]
+    float* acc;              // [batch][ndop][spc_eff]
+    const int32_t* shifts;   // [ndop] drift shifts of this step, sorted-row keyed, in [0, spc)
+    const float* patterns;   // [nsets][Q][ncoh] (column-major (ncoh, Q) per set)
+    const int32_t* pset;     // [nprn] pattern set per PRN
+    const int32_t* req;      // [batch] PRN index of each batch slot
+    const float2* twD;       // column FFT twiddles
+    FftPlanDev pd;
+    int nb, bs, ndop, spc, spc_eff, ncoh, ndb, E, R, Q, ndc, edge_step, use_max;
+    int LB, bstride, first_step;
+};
+
+// Accumulate (or store, at the first step) one power value with the drift
+// shift of its sorted destination row (src/noncoherent_integration.jl:547-569,
+// :728-736; issue #74 keys the shift on the destination row).
+__device__ __forceinline__ void acc_cell(const ColumnArgs& A, int p, int row, int cp, int slice, float v)
+{
+    int dc = cp + A.shifts[row];
+    if (dc >= A.spc) dc -= A.spc;
+    float* dst = A.acc + ((size_t)p * A.ndop + row) * A.spc_eff + (size_t)slice * A.spc + dc;
+    if (A.first_step) *dst = v;
+    else *dst = *dst + v;
+}
+
+// Simple path: column FFT over ndop rows, |x|^2, fftshift fold
+// ((raw + ndop/2) mod ndop, src/plan.jl:528), drift, accumulate.
+// `_accumulate_fftshifted_power_(drift_)pilot!` and `_stream_power_column!`.
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_column_simple(ColumnArgs A)
+{
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x;
+    const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int lag0 = lb * A.LB;
+    const int ncol = min(A.LB, A.bs - lag0);
+    const int ndop = A.ndop, bst = A.bstride;
+    float2* x = lds;
+    float2* y = lds + (size_t)A.LB * bst;
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0;
+    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+        const int k = idx / ncol, c = idx - k * ncol;
+        x[c * bst + k] = T[(size_t)k * A.bs + c];
+    }
+    __syncthreads();
+    float2* res = fft_lds<false>(x, y, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR);
+    const int half = ndop / 2;
+    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+        const int w = idx / ncol, c = idx - w * ncol;
+        const float v = abs2_exact(res[c * bst + w]);
+        int row = w + half;
+        if (row >= ndop) row -= ndop;
+        acc_cell(A, p, row, r * A.bs + lag0 + c, 0, v);
+    }
+}
+
+// Fold raw FFT bins into the sorted stage rows the way the sign-search kernels
+// do (src/noncoherent_integration.jl:92-112, :238-294): raw w < half goes to
+// w + half, raw w >= half to w - half, first loop then second. For odd ndop
+// both raw 0 and raw ndop-1 land on row `half` and row ndop-1 is never
+// written; this reproduces that exactly by iterating destinations.
+template <bool ROT>
+__device__ __forceinline__ void fold_into_stage(float* stage, const float2* res, int ncol, int ndop,
+                                                int bst, int use_max, int tid)
+{
+    const int half = ndop / 2;
+    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+        const int c = idx / ndop, dst = idx - c * ndop;
+        float cur = stage[c * ndop + dst];
+        // first loop: raw w in [0, half) -> dst = w + half
+        if (dst >= half && dst < 2 * half) {
+            const float2 z = res[c * bst + (dst - half)];
+            const float v = ROT ? fmaf(z.x, z.x, __fmul_rn(z.y, z.y)) : abs2_exact(z);
+            if (!use_max) cur = v;
+            else if (v > cur) cur = v;
+        }
+        // second loop: raw w in [half, ndop) -> dst = w - half
+        if (dst < ndop - half) {
+            const float2 z = res[c * bst + (dst + half)];
+            const float v = ROT ? fmaf(z.x, z.x, __fmul_rn(z.y, z.y)) : abs2_exact(z);
+            if (!use_max) cur = v;
+            else if (v > cur) cur = v;
+        }
+        stage[c * ndop + dst] = cur;
+    }
+}
+
+// Scatter a stage (sorted rows) into the accumulator with drift
+// (`_accumulate_sign_tile!` src/noncoherent_integration.jl:695-746).
+__device__ __forceinline__ void scatter_stage(const ColumnArgs& A, const float* stage, int p, int r,
+                                              int lag0, int ncol, int slice, int tid)
+{
+    for (int idx = tid; idx < ncol * A.ndop; idx += FMDBZP_NTHR) {
+        const int row = idx / ncol, c = idx - row * ncol;
+        acc_cell(A, p, row, r * A.bs + lag0 + c, slice, stage[c * A.ndop + row]);
+    }
+}
+
+// Data-bit / bit-edge sign search (`_sign_search_column!`,
+// src/noncoherent_integration.jl:60-115, driven by `_fill_sign_stage_column!`
+// :654-686). For each edge offset and data-bit pattern d the reference sums
+// the pattern-signed FFTs of the N_db zero-padded sub-blocks; by linearity
+// that is the FFT of the signed fold of the sub-blocks onto [0, ndop/N_db),
+// zero-padded, which is what is transformed here (one FFT per pattern).
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_column_sign(ColumnArgs A)
+{
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x;
+    const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int lag0 = lb * A.LB;
+    const int ncol = min(A.LB, A.bs - lag0);
+    const int ndop = A.ndop, bst = A.bstride;
+    float2* x = lds;
+    float2* y = x + (size_t)A.LB * bst;
+    float2* z = y + (size_t)A.LB * bst;
+    float* stage = (float*)(z + (size_t)A.LB * bst);
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0;
+    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+        const int k = idx / ncol, c = idx - k * ncol;
+        x[c * bst + k] = T[(size_t)k * A.bs + c];
+    }
+    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) stage[idx] = 0.f;
+    const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
+    const int sub_len = ndop / A.ndb;
+    const int cpb = A.ncoh / A.ndb;
+    __syncthreads();
+    for (int e = 0; e < A.E; e++) {
+        const int ro = e * A.edge_step;
+        for (int d = 0; d < A.Q; d++) {
+            for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+                const int c = idx / ndop, n = idx - c * ndop;
+                float2 v = make_float2(0.f, 0.f);
+                if (n < sub_len) {
+                    for (int bb = 0; bb < A.ndb; bb++) {
+                        const float sg = pat[(size_t)d * A.ncoh + bb * cpb];
+                        int src = ro + bb * sub_len + n;
+                        src %= ndop;
+                        const float2 u = x[c * bst + src];
+                        v.x += sg * u.x;
+                        v.y += sg * u.y;
+                    }
+                }
+                y[c * bst + n] = v;
+            }
+            __syncthreads();
+            float2* res = fft_lds<false>(y, z, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR);
+            fold_into_stage<false>(stage, res, ncol, ndop, bst, 1, tid);
+            __syncthreads();
+        }
+    }
+    scatter_stage(A, stage, p, r, lag0, ncol, 0, tid);
+}
+
+// Secondary-code rotation search (`_rotation_column!`,
+// src/noncoherent_integration.jl:183-297). The reference combines N_coh
+// zero-padded sub-block FFTs with the phasors patterns[p,q]
+// exp(-2 pi i p s / N_coh) (src/sign_patterns.jl:80-93); that combination is
+// exactly the length-ndop FFT of the column, rotated by the per-column
+// alignment offset and multiplied by the +-1 pattern of coherent period
+// n / nb. That FFT is what is computed here (one FFT per pattern); the +-1
+// multiply is exact in Float32. Patterns sharing a rotation (data-bit combos)
+// and bit-edge offsets max-reduce into that rotation's stage slice.
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_column_rotation(ColumnArgs A)
+{
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x;
+    const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int lag0 = lb * A.LB;
+    const int ncol = min(A.LB, A.bs - lag0);
+    const int ndop = A.ndop, bst = A.bstride;
+    float2* x = lds;
+    float2* y = x + (size_t)A.LB * bst;
+    float2* z = y + (size_t)A.LB * bst;
+    float* stage = (float*)(z + (size_t)A.LB * bst);
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0;
+    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+        const int k = idx / ncol, c = idx - k * ncol;
+        x[c * bst + k] = T[(size_t)k * A.bs + c];
+    }
+    const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
+    const int half_block = A.bs >> 1;
+    __syncthreads();
+    for (int rot = 0; rot < A.R; rot++) {
+        for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) stage[idx] = 0.f;
+        __syncthreads();
+        for (int e = 0; e < A.E; e++) {
+            const int ro = e * A.edge_step;
+            for (int d = 0; d < A.ndc; d++) {
+                const int q = rot * A.ndc + d;
+                const float* pq = pat + (size_t)q * A.ncoh;
+                for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+                    const int c = idx / ndop, n = idx - c * ndop;
+                    // per-column alignment to the primary-code wrap (:212-215)
+                    const int pc = r * A.bs + lag0 + c;
+                    const int wrap = ((A.nb - pc / A.bs) % A.nb) * A.bs + (pc % A.bs);
+                    const int cro = ((wrap + half_block) / A.bs) % A.nb;
+                    const int eff = (ro + cro) % ndop;
+                    int src = eff + n;
+                    if (src >= ndop) src -= ndop;
+                    const float sg = pq[n / A.nb];
+                    const float2 u = x[c * bst + src];
+                    y[c * bst + n] = make_float2(sg * u.x, sg * u.y);
+                }
+                __syncthreads();
+                float2* res = fft_lds<false>(y, z, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR);
+                fold_into_stage<true>(stage, res, ncol, ndop, bst, A.use_max, tid);
+                __syncthreads();
+            }
+        }
+        scatter_stage(A, stage, p, r, lag0, ncol, rot, tid);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions: peak with the reference's tie-break order, FP64 row sums.
+// ---------------------------------------------------------------------------
+// Visiting-order key of cell (row, col_eff). The multistep path and every
+// non-rotation streamed path pick the lowest column, then the lowest row
+// (src/est_signal_noise_power.jl:109-135, src/acquire.jl:352-384); the
+// streamed rotation path visits (code column, rotation slice, row)
+// (src/acquire.jl:311-322, :405-423).
+__device__ __forceinline__ uint64_t cell_key(int row, int col, int ndop, int spc, int R, int streamed_rot)
+{
+    if (streamed_rot) {
+        const int slice = col / spc, cp = col - slice * spc;
+        return ((uint64_t)cp * R + slice) * ndop + row;
+    }
+    return (uint64_t)col * ndop + row;
+}
+
+__device__ __forceinline__ bool better(float v, uint64_t k, float bv, uint64_t bk)
+{
+    return v > bv || (v == bv && k < bk);
+}
+
+// grid (ndop, batch): one workgroup per (row, PRN).
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_reduce_rows(const float* __restrict__ acc, int ndop, int spc, int spc_eff, int R, int streamed_rot,
+              double* __restrict__ rowsum, float* __restrict__ rowbest_v, uint64_t* __restrict__ rowbest_k)
+{
+    __shared__ double ssum[FMDBZP_NTHR];
+    __shared__ float sv[FMDBZP_NTHR];
+    __shared__ uint64_t sk[FMDBZP_NTHR];
+    const int row = blockIdx.x, p = blockIdx.y, tid = threadIdx.x;
+    const float* a = acc + ((size_t)p * ndop + row) * spc_eff;
+    double s = 0.0;
+    float bv = -INFINITY;
+    uint64_t bk = ~0ull;
+    for (int c = tid; c < spc_eff; c += FMDBZP_NTHR) {
+        const float v = a[c];
+        s += (double)v;
+        const uint64_t k = cell_key(row, c, ndop, spc, R, streamed_rot);
+        if (better(v, k, bv, bk)) { bv = v; bk = k; }
+    }
+    ssum[tid] = s;
+    sv[tid] = bv;
+    sk[tid] = bk;
+    __syncthreads();
+    for (int w = FMDBZP_NTHR / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            ssum[tid] += ssum[tid + w];
+            if (better(sv[tid + w], sk[tid + w], sv[tid], sk[tid])) {
+                sv[tid] = sv[tid + w];
+                sk[tid] = sk[tid + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        rowsum[(size_t)p * ndop + row] = ssum[0];
+        rowbest_v[(size_t)p * ndop + row] = sv[0];
+        rowbest_k[(size_t)p * ndop + row] = sk[0];
+    }
+}
+
+// grid (batch): peak, noise estimate (src/acquire.jl:687-697,
+// src/est_signal_noise_power.jl:211-243) and interpolation neighbours.
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
+           const float* __restrict__ rowbest_v, const uint64_t* __restrict__ rowbest_k, int ndop,
+           int spc, int spc_eff, int R, int streamed_rot, int noise_est, int spchip,
+           fmdbzp_prn_stats* __restrict__ stats)
+{
+    __shared__ double ssum[FMDBZP_NTHR];
+    __shared__ float sv[FMDBZP_NTHR];
+    __shared__ uint64_t sk[FMDBZP_NTHR];
+    __shared__ int s_row, s_col;
+    const int p = blockIdx.x, tid = threadIdx.x;
+    float bv = -INFINITY;
+    uint64_t bk = ~0ull;
+    double tot = 0.0;
+    for (int r = tid; r < ndop; r += FMDBZP_NTHR) {
+        const float v = rowbest_v[(size_t)p * ndop + r];
+        const uint64_t k = rowbest_k[(size_t)p * ndop + r];
+        if (better(v, k, bv, bk)) { bv = v; bk = k; }
+        tot += rowsum[(size_t)p * ndop + r];
+    }
+    sv[tid] = bv;
+    sk[tid] = bk;
+    ssum[tid] = tot;
+    __syncthreads();
+    for (int w = FMDBZP_NTHR / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            ssum[tid] += ssum[tid + w];
+            if (better(sv[tid + w], sk[tid + w], sv[tid], sk[tid])) {
+                sv[tid] = sv[tid + w];
+                sk[tid] = sk[tid + w];
+            }
+        }
+        __syncthreads();
+    }
+    const double total = ssum[0];
+    const float peak = sv[0];
+    const uint64_t key = sk[0];
+    if (tid == 0) {
+        int row, col;
+        if (streamed_rot) {
+            row = (int)(key % (uint64_t)ndop);
+            const uint64_t rest = key / (uint64_t)ndop;
+            const int slice = (int)(rest % (uint64_t)R);
+            const int cp = (int)(rest / (uint64_t)R);
+            col = slice * spc + cp;
+        } else {
+            row = (int)(key % (uint64_t)ndop);
+            col = (int)(key / (uint64_t)ndop);
+        }
+        s_row = row;
+        s_col = col;
+    }
+    __syncthreads();
+    const int row = s_row, col = s_col;
+    const float* a = acc + (size_t)p * ndop * spc_eff;
+    // GlobalMean exclusion: columns [col - spchip, col + spchip] clipped, all rows
+    int lo = col - spchip;
+    if (lo < 0) lo = 0;
+    int hi = col + spchip;
+    if (hi > spc_eff - 1) hi = spc_eff - 1;
+    const int nex = hi - lo + 1;
+    double ex = 0.0;
+    if (noise_est == FMDBZP_NOISE_GLOBAL_MEAN) {
+        for (int idx = tid; idx < nex * ndop; idx += FMDBZP_NTHR) {
+            const int rr = idx / nex, cc = lo + idx - rr * nex;
+            ex += (double)a[(size_t)rr * spc_eff + cc];
+        }
+    }
+    __syncthreads();
+    ssum[tid] = ex;
+    __syncthreads();
+    for (int w = FMDBZP_NTHR / 2; w > 0; w >>= 1) {
+        if (tid < w) ssum[tid] += ssum[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        fmdbzp_prn_stats st;
+        st.peak_power = peak;
+        st.peak_row = row;
+        st.peak_col = col;
+        if (noise_est == FMDBZP_NOISE_GLOBAL_MEAN) {
+            const double s = total - ssum[0];
+            const int64_t cnt = (int64_t)(spc_eff - nex) * ndop;
+            st.noise_sum = s;
+            st.noise_cells = cnt;
+            st.noise_power = (float)s / (float)cnt;
+        } else {
+            const int opp = (row + ndop / 2) % ndop;
+            const double s = rowsum[(size_t)p * ndop + opp];
+            st.noise_sum = s;
+            st.noise_cells = spc_eff;
+            st.noise_power = (float)s / (float)spc_eff;
+        }
+        const int rot = col / spc, scr = col - rot * spc;
+        const int cl = (scr == 0 ? spc - 1 : scr - 1) + rot * spc;
+        const int cr = (scr == spc - 1 ? 0 : scr + 1) + rot * spc;
+        const int dl = row == 0 ? ndop - 1 : row - 1;
+        const int dr = row == ndop - 1 ? 0 : row + 1;
+        st.code_left = a[(size_t)row * spc_eff + cl];
+        st.code_right = a[(size_t)row * spc_eff + cr];
+        st.dop_left = a[(size_t)dl * spc_eff + col];
+        st.dop_right = a[(size_t)dr * spc_eff + col];
+        stats[p] = st;
+    }
+}
+
+// Surface export for store_power_bins: acc[row][col] -> Julia column-major
+// (ndop, spc_eff), i.e. out[col * ndop + row]. 32x32 LDS-tiled transpose.
+// grid (ceil(spc_eff/32), ceil(ndop/32)), block (32, 8)
+__global__ void k_export(const float* __restrict__ acc, float* __restrict__ out, int ndop, int spc_eff)
+{
+    __shared__ float t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int i = threadIdx.y; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + threadIdx.x;
+        if (r < ndop && c < spc_eff) t[i][threadIdx.x] = acc[(size_t)r * spc_eff + c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.y; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + threadIdx.x;
+        if (r < ndop && c < spc_eff) out[(size_t)c * ndop + r] = t[threadIdx.x][i];
+    }
+}

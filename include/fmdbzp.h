@@ -1,0 +1,207 @@
+/*
+ * fmdbzp.h — C ABI of the MI355X FM-DBZP acquisition engine (libfmdbzp.so).
+ *
+ * The engine replaces the inner search of Acquisition.jl's `acquire!`: the
+ * PRN-independent prologue (`_downconvert!` + `_precompute_signal_block_ffts!`,
+ * src/acquire.jl:92-104, src/coherent_integration.jl:12-56) and the per-PRN
+ * search (`_acquire_prn_streamed!`, src/acquire.jl:242-327, or
+ * `_accumulate_prn_step_tiled!`, src/noncoherent_integration.jl:760-812,
+ * followed by `est_signal_noise_power`, src/est_signal_noise_power.jl:211-243).
+ * Argument validation, Doppler / code-phase label arithmetic, CN0, the
+ * parabolic interpolation formulas, the secondary-code phase gate and CFAR stay
+ * in the host language above this boundary (src/acquire.jl:544-781).
+ *
+ * Conventions
+ *  - Plain C types only; every pointer is a host pointer unless a flag says
+ *    otherwise. The library never keeps a caller pointer after a call returns.
+ *  - All indices are 0-based (a Julia caller adds 1).
+ *  - Every call returns an int status (FMDBZP_OK == 0, negative on error);
+ *    fmdbzp_last_error() returns a thread-local message for the last failure.
+ *  - A plan handle serialises its own calls (one acquire per plan at a time,
+ *    the reference's contract, src/plan.jl:186-190) and runs on its own HIP
+ *    stream on the device that was current when the plan was created.
+ */
+#ifndef FMDBZP_H
+#define FMDBZP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMDBZP_ABI_VERSION 1
+
+enum fmdbzp_status {
+    FMDBZP_OK = 0,
+    FMDBZP_EINVAL = -1,  /* bad argument or descriptor */
+    FMDBZP_ENOMEM = -2,  /* device or host allocation failed */
+    FMDBZP_EHIP = -3,    /* HIP runtime error */
+    FMDBZP_ENODEV = -4,  /* no usable gfx950 device */
+    FMDBZP_EHANDLE = -5  /* null or destroyed plan handle */
+};
+
+/* Sample formats accepted by fmdbzp_acquire. `ComplexF32(x)` of each sample
+ * is what enters the search (src/acquire.jl:94, :101). */
+enum fmdbzp_eltype {
+    FMDBZP_CF32 = 0,     /* interleaved float re,im (Julia ComplexF32) */
+    FMDBZP_CF64 = 1,     /* interleaved double re,im (Julia ComplexF64) */
+    FMDBZP_I16 = 2,      /* real int16 (Julia Int16 IF samples) */
+    FMDBZP_CI8 = 3,      /* interleaved int8 re,im (Julia Complex{Int8}) */
+    FMDBZP_CI16 = 4,     /* interleaved int16 re,im (Julia Complex{Int16}) */
+    FMDBZP_F32 = 5,      /* real float (Julia Float32) */
+    FMDBZP_F64 = 6       /* real double (Julia Float64) */
+};
+
+/* Search path, as selected by the reference (src/acquire.jl:250-253,
+ * src/noncoherent_integration.jl:769-772). */
+enum fmdbzp_path {
+    FMDBZP_PATH_SIMPLE = 0,    /* N_db == 1, E == 1, no rotation search   */
+    FMDBZP_PATH_SIGN = 1,      /* data-bit / bit-edge sign search          */
+    FMDBZP_PATH_ROTATION = 2   /* secondary-code rotation search           */
+};
+
+enum fmdbzp_noise_estimator {
+    FMDBZP_NOISE_OPPOSITE_ROW = 0,  /* src/est_signal_noise_power.jl:70  */
+    FMDBZP_NOISE_GLOBAL_MEAN = 1    /* src/est_signal_noise_power.jl:100 */
+};
+
+enum fmdbzp_precision {
+    FMDBZP_PREC_FP32 = 0      /* plain FP32 path */
+};
+
+/* fmdbzp_acquire flags */
+#define FMDBZP_FLAG_STORE_SURFACES 0x1u  /* copy (ndop, spc_eff) power surfaces out */
+#define FMDBZP_FLAG_DEVICE_SIGNAL  0x2u  /* `signal` is a device pointer           */
+#define FMDBZP_FLAG_DEVICE_OUTPUT  0x4u  /* surface pointers are device pointers   */
+
+/*
+ * Plan descriptor: every scalar the reference derives in `plan_acquire`
+ * (src/plan.jl:385-528). The host computes these with the reference's exact
+ * arithmetic; the engine only consumes them.
+ */
+typedef struct fmdbzp_plan_desc {
+    int32_t abi_version;             /* FMDBZP_ABI_VERSION                          */
+    int32_t samples_per_code;        /* spc, src/plan.jl:386                        */
+    int32_t num_blocks;              /* nb,  src/plan.jl:418-426                    */
+    int32_t block_size;              /* bs = spc / nb, src/plan.jl:448              */
+    int32_t fft_size;                /* F = nextprod((2,3,5,7), 2 bs), :460         */
+    int32_t num_coh;                 /* N_coh                                       */
+    int32_t num_data_bits;           /* N_db, src/plan.jl:464                       */
+    int32_t bit_edge_search_steps;   /* E                                           */
+    int32_t num_noncoherent;         /* N_nc                                        */
+    int32_t num_secondary_rotations; /* R, src/plan.jl:511-515                      */
+    int32_t num_doppler_bins;        /* ndop = N_coh * nb, src/plan.jl:465          */
+    int32_t samples_per_code_eff;    /* spc * R (rotation) or spc, :524-525         */
+    int32_t num_patterns;            /* Q columns of the sign-pattern matrix        */
+    int32_t path;                    /* enum fmdbzp_path                            */
+    int32_t noise_estimator;         /* enum fmdbzp_noise_estimator                 */
+    int32_t samples_per_chip;        /* floor(fs / f_code), GlobalMean exclusion    */
+    int32_t precision;               /* enum fmdbzp_precision                       */
+    int32_t reserved[7];
+    double sampling_freq_hz;
+} fmdbzp_plan_desc;
+
+/*
+ * Per-PRN search statistics returned by fmdbzp_acquire: what the reference's
+ * per-PRN search hands to result extraction (src/acquire.jl:202-209 and
+ * :549-555), plus the four neighbour cells that parabolic interpolation reads
+ * (src/acquire.jl:597-618, :718-737), so no surface has to cross the bus.
+ */
+typedef struct fmdbzp_prn_stats {
+    float peak_power;      /* max of the (ndop, spc_eff) power surface            */
+    int32_t peak_row;      /* 0-based sorted-Doppler row of the peak              */
+    int32_t peak_col;      /* 0-based effective column (rotation * spc + col)     */
+    float noise_power;     /* estimator output (OppositeRow or GlobalMean)         */
+    float code_left;       /* (peak_row, scrambled col - 1 mod spc, same rotation) */
+    float code_right;      /* (peak_row, scrambled col + 1 mod spc, same rotation) */
+    float dop_left;        /* (peak_row - 1 wrapping, peak_col)                    */
+    float dop_right;       /* (peak_row + 1 wrapping, peak_col)                    */
+    double noise_sum;      /* FP64 sum of the cells averaged into noise_power      */
+    int64_t noise_cells;   /* number of those cells                                */
+} fmdbzp_prn_stats;
+
+typedef struct fmdbzp_plan fmdbzp_plan;  /* opaque, device-resident plan */
+
+/*
+ * Create a device-resident plan. Replaces plan.jl's device-independent tables:
+ * the conjugated PRN sub-block FFTs (`_precompute_prn_ffts!`,
+ * src/plan.jl:230-251, computed on the device from the code samples given
+ * here), the sign patterns (src/plan.jl:627-672) and the per-step code-drift
+ * shifts (`_fill_code_drift_shifts!`, src/noncoherent_integration.jl:600-619,
+ * evaluated by the host in FP64).
+ *
+ *  code_samples   nprn x spc int8 (+1/-1): gen_code(spc, system, prn, fs, f_code, 0)
+ *  patterns       npattern_sets x (N_coh x Q) float, column-major like the
+ *                 reference's Matrix{Float32} (row = coherent period)
+ *  pattern_set    nprn int32: which pattern set each PRN uses (may be NULL
+ *                 when npattern_sets <= 1)
+ *  drift_shifts   N_nc x ndop int32 in [0, spc), indexed [step][sorted row]
+ */
+int fmdbzp_plan_create(const fmdbzp_plan_desc* desc,
+                       int32_t nprn,
+                       const int8_t* code_samples,
+                       int32_t npattern_sets,
+                       const float* patterns,
+                       const int32_t* pattern_set,
+                       const int32_t* drift_shifts,
+                       fmdbzp_plan** out_plan);
+
+/*
+ * Run the search for `nreq` PRNs of the plan (indices into the plan's PRN
+ * list) over `signal` (n_samples samples of `eltype`, at least N_nc segments
+ * of N_coh * spc samples, src/acquire.jl:65-72). Replaces the body of
+ * `acquire!` from `_downconvert!` to the per-PRN statistics
+ * (src/acquire.jl:114-215).
+ *
+ *  out_stats      nreq entries
+ *  out_surfaces   NULL, or nreq pointers to (ndop, spc_eff) float surfaces in
+ *                 Julia column-major order (row fastest), written when
+ *                 FMDBZP_FLAG_STORE_SURFACES is set (store_power_bins).
+ */
+int fmdbzp_acquire(fmdbzp_plan* plan,
+                   const void* signal,
+                   int32_t eltype,
+                   int64_t n_samples,
+                   double interm_freq_hz,
+                   const int32_t* prn_index,
+                   int32_t nreq,
+                   uint32_t flags,
+                   fmdbzp_prn_stats* out_stats,
+                   float* const* out_surfaces);
+
+/* Release every device allocation of the plan. NULL is accepted. */
+int fmdbzp_plan_destroy(fmdbzp_plan* plan);
+
+/* Thread-local text of the last error ("" when none). */
+const char* fmdbzp_last_error(void);
+
+/* Device and build facts: writes a NUL-terminated string into buf. */
+int fmdbzp_query(char* buf, size_t buflen);
+
+/*
+ * Optional per-kernel timing for measurement: when enabled, the plan records
+ * HIP events around every kernel launch on its own stream and accumulates the
+ * elapsed time per kernel class. fmdbzp_plan_kernel_times fills up to `n`
+ * entries of total milliseconds and launch counts, in the order of
+ * enum fmdbzp_kernel_class, and returns the number of classes.
+ */
+enum fmdbzp_kernel_class {
+    FMDBZP_K_CODE_FFT = 0,    /* plan-time conj code FFTs                 */
+    FMDBZP_K_SIGNAL_FFT = 1,  /* downconvert + signal double-block FFTs   */
+    FMDBZP_K_CORR_TILE = 2,   /* block correlation: multiply + inverse FFT */
+    FMDBZP_K_COLUMN = 3,      /* column Doppler FFT / sign search + accumulate */
+    FMDBZP_K_REDUCE = 4,      /* per-row sums and peak candidates          */
+    FMDBZP_K_FINALIZE = 5,    /* per-PRN peak, noise, neighbours           */
+    FMDBZP_K_EXPORT = 6,      /* surface transpose for store_power_bins    */
+    FMDBZP_K_NUM_CLASSES = 7
+};
+int fmdbzp_plan_set_timing(fmdbzp_plan* plan, int32_t enable);
+int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launches, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMDBZP_H */

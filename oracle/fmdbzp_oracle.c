@@ -1,0 +1,1148 @@
+/*
+ * fmdbzp_oracle.c — CPU restatement of the FM-DBZP search of
+ * JuliaGNSS/Acquisition.jl v2.6.4 (file:line citations below are relative to
+ * that package's source tree).
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the parity checker for the HIP
+ * engine: only tests, the smoke check and the benchmark's CPU-baseline leg
+ * load it. The product never links, calls or falls back to it.
+ *
+ * It is compiled twice from this one source:
+ *   REAL=float   parity mode: Float32 storage and arithmetic where the
+ *                reference uses Float32 (the reference's own precision);
+ *   REAL=double  FP64 mode: the same algorithm with every intermediate kept in
+ *                double, used to state the error of the FP32 engine.
+ *
+ * The reference's FFTs come from FFTW (not vendored, Julia package FFTW.jl
+ * compat "1.0"); this restatement uses its own mixed-radix decimation-in-time
+ * FFT with FP64-computed twiddles, which agrees with FFTW to FP32 rounding.
+ * Loop orders, index maps, fold directions, tie-break order and the Float32
+ * formula quirks follow the reference exactly; summation order of the
+ * `@simd` reductions is sequential here (the reference leaves it unspecified).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#include "../include/fmdbzp.h"
+
+#ifndef REAL
+#define REAL float
+#endif
+typedef REAL real;
+#define IS_F32 (sizeof(real) == sizeof(float))
+
+typedef struct { real re, im; } cpx;
+
+/* ------------------------------------------------------------------------ */
+/* sinpi / cospi with exact results at multiples of 1/2 (Julia's cispi).    */
+/* ------------------------------------------------------------------------ */
+static void sincospi_d(double x, double* s, double* c)
+{
+    /* reduce to r in [-1, 1]: x - 2 round(x/2) is exact in binary FP */
+    double r = x - 2.0 * nearbyint(x * 0.5);
+    /* fold to [-1/2, 1/2] using sin(pi (1 - r)) = sin(pi r), cos(..) = -cos */
+    double sgn_c = 1.0;
+    if (r > 0.5) { r = 1.0 - r; sgn_c = -1.0; }
+    else if (r < -0.5) { r = -1.0 - r; sgn_c = -1.0; }
+    /* now |r| <= 1/2: split at 1/4 for accuracy */
+    double ar = fabs(r);
+    double ss, cc;
+    if (ar <= 0.25) {
+        ss = sin(M_PI * ar);
+        cc = cos(M_PI * ar);
+    } else {
+        double t = 0.5 - ar;  /* exact */
+        ss = cos(M_PI * t);
+        cc = sin(M_PI * t);
+    }
+    if (r < 0) ss = -ss;
+    *s = ss;
+    *c = sgn_c * cc;
+    if (x == 0.0 || r == 0.0) *s = (r == 0.0 && sgn_c < 0) ? 0.0 : *s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT: recursive mixed-radix decimation in time, unnormalised.             */
+/* ------------------------------------------------------------------------ */
+#define MAXFAC 40
+typedef struct {
+    int n;
+    int nf;
+    int fac[MAXFAC];
+    cpx* tw;        /* tw[m] = exp(-2 pi i m / n) */
+} offt;
+
+static int offt_init(offt* p, int n)
+{
+    p->n = n;
+    p->nf = 0;
+    int m = n;
+    while (m % 4 == 0) { p->fac[p->nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { p->fac[p->nf++] = 2; m /= 2; }
+    for (int f = 3; m > 1; f += 2) {
+        while (m % f == 0) { p->fac[p->nf++] = f; m /= f; }
+    }
+    p->tw = (cpx*)malloc(sizeof(cpx) * (size_t)n);
+    if (!p->tw) return -1;
+    for (int k = 0; k < n; k++) {
+        double s, c;
+        sincospi_d((2.0 * (double)k) / (double)n, &s, &c);
+        p->tw[k].re = (real)c;
+        p->tw[k].im = (real)(-s);
+    }
+    return 0;
+}
+
+static void offt_free(offt* p)
+{
+    free(p->tw);
+    p->tw = NULL;
+}
+
+static inline cpx cmulr(cpx a, cpx b)
+{
+    cpx r;
+    r.re = a.re * b.re - a.im * b.im;
+    r.im = a.re * b.im + a.im * b.re;
+    return r;
+}
+
+static void fft_rec(const offt* P, int n, int fi, const cpx* in, int is, cpx* out,
+                    int ts, int inv, cpx* t)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    const int r = P->fac[fi];
+    const int m = n / r;
+    for (int q = 0; q < r; q++)
+        fft_rec(P, m, fi + 1, in + (size_t)q * is, is * r, out + (size_t)q * m, ts * r, inv, t);
+    for (int k = 0; k < m; k++) {
+        t[0] = out[k];
+        for (int q = 1; q < r; q++) {
+            cpx w = P->tw[(size_t)q * k * ts];
+            if (inv) w.im = -w.im;
+            t[q] = cmulr(out[(size_t)q * m + k], w);
+        }
+        if (r == 2) {
+            cpx a = t[0], b = t[1];
+            out[k].re = a.re + b.re; out[k].im = a.im + b.im;
+            out[k + m].re = a.re - b.re; out[k + m].im = a.im - b.im;
+        } else if (r == 4) {
+            cpx a, b, c, d;
+            a.re = t[0].re + t[2].re; a.im = t[0].im + t[2].im;
+            b.re = t[0].re - t[2].re; b.im = t[0].im - t[2].im;
+            c.re = t[1].re + t[3].re; c.im = t[1].im + t[3].im;
+            d.re = t[1].re - t[3].re; d.im = t[1].im - t[3].im;
+            /* forward: -i*d = (d.im, -d.re); inverse: +i*d = (-d.im, d.re) */
+            cpx jd;
+            if (!inv) { jd.re = d.im; jd.im = -d.re; }
+            else { jd.re = -d.im; jd.im = d.re; }
+            out[k].re = a.re + c.re; out[k].im = a.im + c.im;
+            out[k + 2 * m].re = a.re - c.re; out[k + 2 * m].im = a.im - c.im;
+            out[k + m].re = b.re + jd.re; out[k + m].im = b.im + jd.im;
+            out[k + 3 * m].re = b.re - jd.re; out[k + 3 * m].im = b.im - jd.im;
+        } else {
+            for (int s = 0; s < r; s++) {
+                cpx acc = t[0];
+                for (int q = 1; q < r; q++) {
+                    cpx w = P->tw[(size_t)((q * s) % r) * m * ts];
+                    if (inv) w.im = -w.im;
+                    cpx v = cmulr(t[q], w);
+                    acc.re += v.re;
+                    acc.im += v.im;
+                }
+                t[r + s] = acc;
+            }
+            for (int s = 0; s < r; s++) out[k + (size_t)m * s] = t[r + s];
+        }
+    }
+}
+
+/* In-place transform of data[0..n): inv = 0 forward (FFTW `fft`), 1 backward
+ * unnormalised (FFTW `bfft`). work needs 3n entries. */
+static void offt_exec(const offt* P, cpx* data, cpx* work, int inv)
+{
+    memcpy(work, data, sizeof(cpx) * (size_t)P->n);
+    fft_rec(P, P->n, 0, work, 1, data, 1, inv, work + P->n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sample access: ComplexF32(x) of each input sample (src/acquire.jl:94,101) */
+/* ------------------------------------------------------------------------ */
+static inline void load_sample(const void* sig, int eltype, int64_t i, float* re, float* im)
+{
+    switch (eltype) {
+    case FMDBZP_CF32: { const float* p = (const float*)sig; *re = p[2 * i]; *im = p[2 * i + 1]; break; }
+    case FMDBZP_CF64: { const double* p = (const double*)sig; *re = (float)p[2 * i]; *im = (float)p[2 * i + 1]; break; }
+    case FMDBZP_I16: { const int16_t* p = (const int16_t*)sig; *re = (float)p[i]; *im = 0.0f; break; }
+    case FMDBZP_CI8: { const int8_t* p = (const int8_t*)sig; *re = (float)p[2 * i]; *im = (float)p[2 * i + 1]; break; }
+    case FMDBZP_CI16: { const int16_t* p = (const int16_t*)sig; *re = (float)p[2 * i]; *im = (float)p[2 * i + 1]; break; }
+    case FMDBZP_F32: { const float* p = (const float*)sig; *re = p[i]; *im = 0.0f; break; }
+    case FMDBZP_F64: { const double* p = (const double*)sig; *re = (float)p[i]; *im = 0.0f; break; }
+    default: *re = 0.0f; *im = 0.0f;
+    }
+}
+
+/* `_downconvert!` src/acquire.jl:92-104. The IF phase argument is formed in
+ * Float32 exactly as the reference does (Float32 multiply then Float32 add,
+ * never fused); sin/cos of that Float32 argument are evaluated in double and
+ * rounded, as Julia's Float32 `sincos` does. */
+static void o_downconvert(cpx* out, const void* sig, int eltype, int64_t seg0, int64_t seglen,
+                          double if_hz, double fs)
+{
+    if (if_hz == 0.0) {
+        for (int64_t n = 0; n < seglen; n++) {
+            float re, im;
+            load_sample(sig, eltype, seg0 + n, &re, &im);
+            out[n].re = (real)re;
+            out[n].im = (real)im;
+        }
+        return;
+    }
+    const double twopi = 6.283185307179586;  /* Julia `2π` */
+    volatile float step = (float)((-twopi * if_hz) / fs);
+    volatile float off = (float)seg0 * step;
+    for (int64_t n = 0; n < seglen; n++) {
+        volatile float prod = (float)n * step;
+        volatile float phase = off + prod;
+        double ph = (double)phase;
+        real s = (real)sin(ph), c = (real)cos(ph);
+        float re, im;
+        load_sample(sig, eltype, seg0 + n, &re, &im);
+        real a = (real)re, b = (real)im;
+        out[n].re = a * c - b * s;
+        out[n].im = a * s + b * c;
+    }
+}
+
+/* `_precompute_signal_block_ffts!` src/coherent_integration.jl:12-56:
+ * double block k = samples [k bs, k bs + 2 bs) of the segment, the last one
+ * wrapping to the segment start (:44-50), zero-padded to F, forward FFT. */
+static void o_signal_block_ffts(cpx* cache, const cpx* seg, int spc, int nb, int bs, int ncoh,
+                                const offt* PF, cpx* work)
+{
+    const int F = PF->n;
+    const int64_t seglen = (int64_t)ncoh * spc;
+    cpx* buf = work + 3 * F;
+    for (int gk = 0; gk < ncoh * nb; gk++) {
+        for (int i = 0; i < F; i++) { buf[i].re = 0; buf[i].im = 0; }
+        int64_t bstart = (int64_t)gk * bs;
+        int64_t nstart = (int64_t)(gk + 1) * bs;
+        if (nstart + bs <= seglen) {
+            for (int i = 0; i < bs; i++) buf[i] = seg[bstart + i];
+            for (int i = 0; i < bs; i++) buf[bs + i] = seg[nstart + i];
+        } else {
+            int64_t remaining = seglen - nstart;
+            for (int i = 0; i < bs; i++) buf[i] = seg[bstart + i];
+            for (int64_t i = 0; i < remaining; i++) buf[bs + i] = seg[nstart + i];
+            for (int64_t i = 0; i < bs - remaining; i++) buf[bs + remaining + i] = seg[i];
+        }
+        offt_exec(PF, buf, work, 0);
+        memcpy(cache + (size_t)gk * F, buf, sizeof(cpx) * (size_t)F);
+    }
+}
+
+/* `_precompute_prn_ffts!` src/plan.jl:230-251: conj(FFT(sub-block k of the
+ * code, zero-padded to F)), stored (F, nb) column-major. */
+static void o_prn_conj_ffts(cpx* out, const int8_t* code, int nb, int bs, const offt* PF, cpx* work)
+{
+    const int F = PF->n;
+    cpx* buf = work + 3 * F;
+    for (int b = 0; b < nb; b++) {
+        for (int i = 0; i < F; i++) { buf[i].re = 0; buf[i].im = 0; }
+        for (int i = 0; i < bs; i++) buf[i].re = (real)code[(size_t)b * bs + i];
+        offt_exec(PF, buf, work, 0);
+        for (int i = 0; i < F; i++) {
+            out[(size_t)b * F + i].re = buf[i].re;
+            out[(size_t)b * F + i].im = -buf[i].im;
+        }
+    }
+}
+
+/* `_build_coherent_tile!` src/coherent_integration.jl:136-171. tile is the
+ * Julia (ndop, bs) column-major matrix: tile[lag * ndop + row]. */
+static void o_build_tile(cpx* tile, const cpx* cache, const cpx* prnconj, int r, int nb, int bs,
+                         int ncoh, const offt* PF, cpx* work)
+{
+    const int F = PF->n;
+    const int ndop = ncoh * nb;
+    const real inv_f = (real)1 / (real)F;       /* `1f0 / length(corr_buf)` */
+    cpx* buf = work + 3 * F;
+    for (int p = 0; p < ncoh; p++) {
+        for (int b = 0; b < nb; b++) {
+            int gk = p * nb + b;
+            int pb = (b + r) % nb;
+            const cpx* s = cache + (size_t)gk * F;
+            const cpx* c = prnconj + (size_t)pb * F;
+            for (int i = 0; i < F; i++) buf[i] = cmulr(s[i], c[i]);
+            offt_exec(PF, buf, work, 1);
+            for (int lag = 0; lag < bs; lag++) {
+                tile[(size_t)lag * ndop + gk].re = buf[lag].re * inv_f;
+                tile[(size_t)lag * ndop + gk].im = buf[lag].im * inv_f;
+            }
+        }
+    }
+}
+
+static inline real abs2r(cpx z) { return z.re * z.re + z.im * z.im; }
+
+/* ------------------------------------------------------------------------ */
+/* Plan and per-thread scratch                                               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    fmdbzp_plan_desc d;
+    int nprn;
+    cpx* prnconj;          /* nprn x nb x F */
+    real* patterns;        /* nprn x (ncoh x Q), column-major per PRN */
+    real* tiled_re;        /* nprn x (ndop x ncoh x Q) */
+    real* tiled_im;
+    const int32_t* shifts; /* nnc x ndop */
+    offt PF, PD;
+} oplan;
+
+typedef struct {
+    cpx* work;       /* FFT scratch (3 max(F,ndop) + max(F,ndop)) */
+    cpx* tile;       /* ndop x bs */
+    cpx* colbuf;     /* ndop */
+    cpx* sub;        /* ndop x max(ndb, ncoh) */
+    real* cre;       /* ndop */
+    real* cim;       /* ndop */
+    real* stage;     /* ndop x R */
+} oscratch;
+
+/* `sign_patterns` src/sign_patterns.jl:23-47 (column-major (ncoh, Q)). */
+void oracle_sign_patterns(const int8_t* sec, int prn_sec_len, int ndb, int R, int ncoh,
+                          int use_sec, float* out)
+{
+    int ndc = 1 << (ndb - 1);
+    int sec_active = use_sec && R > 1;
+    int nsec = sec_active ? R : 1;
+    int cpb = ncoh / ndb;
+    int L = sec_active ? R : 1;
+    int col = 0;
+    (void)prn_sec_len;
+    for (int r = 0; r < nsec; r++) {
+        for (int d = 0; d < ndc; d++) {
+            for (int k = 0; k < ncoh; k++) {
+                int bit = k / cpb;
+                float ds = bit == 0 ? 1.0f : ((((d >> (bit - 1)) & 1) == 0) ? 1.0f : -1.0f);
+                float ss = sec_active ? (float)sec[(k + r) % L] : 1.0f;
+                out[(size_t)col * ncoh + k] = ds * ss;
+            }
+            col++;
+        }
+    }
+}
+
+/* `combined_phase_patterns` + `tile_phase_patterns` src/sign_patterns.jl:80-129:
+ * tiled[w, p, q] = patterns[p, q] * cispi(-2 p s / ncoh), s = w mod ncoh,
+ * the phase formed in Float32 as the reference does (:89). */
+static void o_tiled_phase(const real* pat, int ncoh, int Q, int ndop, real* tre, real* tim)
+{
+    for (int q = 0; q < Q; q++)
+        for (int p = 0; p < ncoh; p++)
+            for (int w = 0; w < ndop; w++) {
+                int s = w % ncoh;
+                double sx, cx;
+                if (IS_F32) {
+                    volatile float a = -2.0f * (float)p;
+                    volatile float b2 = a * (float)s;
+                    volatile float x = b2 / (float)ncoh;
+                    sincospi_d((double)x, &sx, &cx);
+                    float cr = (float)cx, ci = (float)sx;
+                    float pv = (float)pat[(size_t)q * ncoh + p];
+                    tre[((size_t)q * ncoh + p) * ndop + w] = (real)(pv * cr);
+                    tim[((size_t)q * ncoh + p) * ndop + w] = (real)(pv * ci);
+                } else {
+                    double x = (-2.0 * p * s) / ncoh;
+                    sincospi_d(x, &sx, &cx);
+                    double pv = (double)pat[(size_t)q * ncoh + p];
+                    tre[((size_t)q * ncoh + p) * ndop + w] = (real)(pv * cx);
+                    tim[((size_t)q * ncoh + p) * ndop + w] = (real)(pv * sx);
+                }
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Column stages                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* `_sign_search_column!` src/noncoherent_integration.jl:60-115. */
+static void o_sign_search_column(real* stage_col, const cpx* tile, int src_col, int ndop, int ndb,
+                                 int row_offset, const real* pat, int Q, int ncoh, const offt* PD,
+                                 oscratch* S)
+{
+    int sub_len = ndop / ndb;
+    int cpb = ncoh / ndb;
+    const cpx* col = tile + (size_t)src_col * ndop;
+    for (int b = 0; b < ndb; b++) {
+        cpx* cb = S->colbuf;
+        for (int i = 0; i < ndop; i++) { cb[i].re = 0; cb[i].im = 0; }
+        for (int sr = 0; sr < sub_len; sr++) {
+            int src_row = (row_offset + b * sub_len + sr) % ndop;
+            cb[sr] = col[src_row];
+        }
+        offt_exec(PD, cb, S->work, 0);
+        memcpy(S->sub + (size_t)b * ndop, cb, sizeof(cpx) * (size_t)ndop);
+    }
+    int half = ndop / 2;
+    for (int p = 0; p < Q; p++) {
+        cpx* cb = S->colbuf;
+        for (int i = 0; i < ndop; i++) { cb[i].re = 0; cb[i].im = 0; }
+        for (int b = 0; b < ndb; b++) {
+            real sgn = pat[(size_t)p * ncoh + b * cpb];
+            const cpx* sb = S->sub + (size_t)b * ndop;
+            for (int i = 0; i < ndop; i++) {
+                cb[i].re += sgn * sb[i].re;
+                cb[i].im += sgn * sb[i].im;
+            }
+        }
+        for (int w = 0; w < half; w++) {
+            real v = abs2r(cb[w]);
+            int dst = w + half;
+            if (v > stage_col[dst]) stage_col[dst] = v;
+        }
+        for (int w = half; w < ndop; w++) {
+            real v = abs2r(cb[w]);
+            int dst = w - half;
+            if (v > stage_col[dst]) stage_col[dst] = v;
+        }
+    }
+}
+
+/* `_rotation_column!` src/noncoherent_integration.jl:183-297. stage is
+ * (ndop, R) column-major. cp_col is 0-based. */
+static void o_rotation_column(real* stage, const cpx* tile, int src_col, int cp_col, int ndop,
+                              int ncoh, int ndc, int nb, int bs, int row_offset, const real* tre,
+                              const real* tim, int Q, int use_max, const offt* PD, oscratch* S)
+{
+    int half_block = bs >> 1;
+    int pc = cp_col;
+    int wrap_sample = ((nb - pc / bs) % nb) * bs + (pc % bs);
+    int col_row_offset = ((wrap_sample + half_block) / bs) % nb;
+    int eff = (row_offset + col_row_offset) % ndop;
+    const cpx* col = tile + (size_t)src_col * ndop;
+    for (int k = 0; k < ncoh; k++) {
+        cpx* cb = S->colbuf;
+        for (int i = 0; i < ndop; i++) { cb[i].re = 0; cb[i].im = 0; }
+        for (int sr = 0; sr < nb; sr++) {
+            int src_row = (eff + k * nb + sr) % ndop;
+            cb[sr] = col[src_row];
+        }
+        offt_exec(PD, cb, S->work, 0);
+        memcpy(S->sub + (size_t)k * ndop, cb, sizeof(cpx) * (size_t)ndop);
+    }
+    int half = ndop / 2;
+    for (int q = 0; q < Q; q++) {
+        for (int w = 0; w < ndop; w++) { S->cre[w] = 0; S->cim[w] = 0; }
+        for (int p = 0; p < ncoh; p++) {
+            const real* ar_ = tre + ((size_t)q * ncoh + p) * ndop;
+            const real* ai_ = tim + ((size_t)q * ncoh + p) * ndop;
+            const cpx* sb = S->sub + (size_t)p * ndop;
+            for (int w = 0; w < ndop; w++) {
+                real ar = ar_[w], ai = ai_[w];
+                real br = sb[w].re, bi = sb[w].im;
+                if (IS_F32) {
+                    S->cre[w] = (real)fmaf((float)ar, (float)br, fmaf(-(float)ai, (float)bi, (float)S->cre[w]));
+                    S->cim[w] = (real)fmaf((float)ar, (float)bi, fmaf((float)ai, (float)br, (float)S->cim[w]));
+                } else {
+                    S->cre[w] = fma(ar, br, fma(-ai, bi, S->cre[w]));
+                    S->cim[w] = fma(ar, bi, fma(ai, br, S->cim[w]));
+                }
+            }
+        }
+        int dst_col = q / ndc;
+        real* st = stage + (size_t)dst_col * ndop;
+        for (int w = 0; w < ndop; w++) {
+            real re = S->cre[w], im = S->cim[w];
+            real v = IS_F32 ? (real)fmaf((float)re, (float)re, (float)(im * im)) : fma(re, re, im * im);
+            int dst = (w < half) ? w + half : w - half;
+            if (!use_max) st[dst] = v;
+            else if (v > st[dst]) st[dst] = v;
+        }
+    }
+}
+
+/* `_fill_sign_stage_column!` src/noncoherent_integration.jl:654-686. */
+static void o_fill_sign_stage_column(real* stage, const cpx* tile, int src_col, int cp_col,
+                                     const oplan* P, int prn, oscratch* S)
+{
+    const fmdbzp_plan_desc* d = &P->d;
+    int ndop = d->num_doppler_bins;
+    int rot = d->path == FMDBZP_PATH_ROTATION;
+    int ndc = 1 << (d->num_data_bits - 1);
+    int bpc = d->num_coh / d->num_data_bits;
+    int edge_step = (bpc * d->num_blocks) / d->bit_edge_search_steps;
+    int use_max = ndc > 1 || d->bit_edge_search_steps > 1;
+    int Q = d->num_patterns;
+    size_t tsz = (size_t)ndop * d->num_coh * Q;
+    for (int e = 0; e < d->bit_edge_search_steps; e++) {
+        int row_offset = e * edge_step;
+        if (rot) {
+            o_rotation_column(stage, tile, src_col, cp_col, ndop, d->num_coh, ndc, d->num_blocks,
+                              d->block_size, row_offset, P->tiled_re + (size_t)prn * tsz,
+                              P->tiled_im + (size_t)prn * tsz, Q, use_max, &P->PD, S);
+        } else {
+            o_sign_search_column(stage, tile, src_col, ndop, d->num_data_bits, row_offset,
+                                 P->patterns + (size_t)prn * d->num_coh * Q, Q, d->num_coh, &P->PD, S);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Streamed path (N_nc == 1): `_acquire_prn_streamed!` src/acquire.jl:242-327 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    real peak;
+    int row, col;      /* 0-based */
+} opeak;
+
+static void o_acquire_prn_streamed(const oplan* P, int prn, const cpx* cache, real* surface,
+                                   real* row_sums, real* col_sums, oscratch* S, opeak* out)
+{
+    const fmdbzp_plan_desc* d = &P->d;
+    int ndop = d->num_doppler_bins, bs = d->block_size, nb = d->num_blocks, spc = d->samples_per_code;
+    const cpx* prnconj = P->prnconj + (size_t)prn * nb * d->fft_size;
+    for (int i = 0; i < ndop; i++) row_sums[i] = 0;
+    real peak = -INFINITY;
+    int prow = 0, pcol = 0;
+    if (d->path == FMDBZP_PATH_SIMPLE) {
+        int shift = ndop / 2, front = ndop - shift;
+        for (int r = 0; r < nb; r++) {
+            o_build_tile(S->tile, cache, prnconj, r, nb, bs, d->num_coh, &P->PF, S->work);
+            for (int lag = 0; lag < bs; lag++) {
+                cpx* x = S->colbuf;
+                memcpy(x, S->tile + (size_t)lag * ndop, sizeof(cpx) * (size_t)ndop);
+                offt_exec(&P->PD, x, S->work, 0);
+                int dest_col = r * bs + lag;
+                /* `_stream_power_column!` src/acquire.jl:335-385 */
+                real col_sum = 0, col_max = -INFINITY;
+                int col_max_row = 0;
+                for (int w = front; w < ndop; w++) {
+                    real v = abs2r(x[w]);
+                    int dst = w - front;
+                    row_sums[dst] += v;
+                    col_sum += v;
+                    if (v > col_max) { col_max = v; col_max_row = dst; }
+                    surface[(size_t)dest_col * ndop + dst] = v;
+                }
+                for (int w = 0; w < front; w++) {
+                    real v = abs2r(x[w]);
+                    int dst = w + shift;
+                    row_sums[dst] += v;
+                    col_sum += v;
+                    if (v > col_max) { col_max = v; col_max_row = dst; }
+                    surface[(size_t)dest_col * ndop + dst] = v;
+                }
+                col_sums[dest_col] = col_sum;
+                if (col_max > peak) { peak = col_max; prow = col_max_row; pcol = dest_col; }
+            }
+        }
+    } else {
+        int R = d->num_secondary_rotations;
+        for (int r = 0; r < nb; r++) {
+            o_build_tile(S->tile, cache, prnconj, r, nb, bs, d->num_coh, &P->PF, S->work);
+            for (int lag = 0; lag < bs; lag++) {
+                int cp_col = r * bs + lag;
+                for (size_t i = 0; i < (size_t)ndop * R; i++) S->stage[i] = 0;
+                o_fill_sign_stage_column(S->stage, S->tile, lag, cp_col, P, prn, S);
+                for (int s = 0; s < R; s++) {
+                    /* `_stream_stage_column!` src/acquire.jl:390-424 */
+                    const real* sc = S->stage + (size_t)s * ndop;
+                    int dest_col = cp_col + s * spc;
+                    real col_sum = 0, col_max = -INFINITY;
+                    int col_max_row = 0;
+                    for (int w = 0; w < ndop; w++) {
+                        real v = sc[w];
+                        row_sums[w] += v;
+                        col_sum += v;
+                        if (v > col_max) { col_max = v; col_max_row = w; }
+                        surface[(size_t)dest_col * ndop + w] = v;
+                    }
+                    col_sums[dest_col] = col_sum;
+                    if (col_max > peak) { peak = col_max; prow = col_max_row; pcol = dest_col; }
+                }
+            }
+        }
+    }
+    out->peak = peak;
+    out->row = prow;
+    out->col = pcol;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multistep path (N_nc > 1): `_accumulate_prn_step_tiled!`                  */
+/* src/noncoherent_integration.jl:760-812                                    */
+/* ------------------------------------------------------------------------ */
+static void o_accumulate_prn_step(const oplan* P, int prn, const cpx* cache, real* acc, int step,
+                                  oscratch* S)
+{
+    const fmdbzp_plan_desc* d = &P->d;
+    int ndop = d->num_doppler_bins, bs = d->block_size, nb = d->num_blocks, spc = d->samples_per_code;
+    const cpx* prnconj = P->prnconj + (size_t)prn * nb * d->fft_size;
+    const int32_t* shifts = P->shifts + (size_t)step * ndop;
+    int has_drift = 0;
+    for (int i = 0; i < ndop; i++) if (shifts[i] != 0) has_drift = 1;
+    for (int r = 0; r < nb; r++) {
+        o_build_tile(S->tile, cache, prnconj, r, nb, bs, d->num_coh, &P->PF, S->work);
+        int off = r * bs;
+        if (d->path == FMDBZP_PATH_SIMPLE) {
+            /* `_accumulate_fftshifted_power_pilot!` :482-504 and its drift
+             * counterpart :547-569 (destination-row-keyed shift, issue #74) */
+            for (int lag = 0; lag < bs; lag++) {
+                cpx* x = S->colbuf;
+                memcpy(x, S->tile + (size_t)lag * ndop, sizeof(cpx) * (size_t)ndop);
+                offt_exec(&P->PD, x, S->work, 0);
+                int c = off + lag;
+                for (int w = 0; w < ndop; w++) {
+                    int dst_row = (w + ndop / 2) % ndop;
+                    int sum = c + (has_drift ? shifts[dst_row] : 0);
+                    int dst_col = sum >= spc ? sum - spc : sum;
+                    acc[(size_t)dst_col * ndop + dst_row] += abs2r(x[w]);
+                }
+            }
+        } else {
+            /* `_accumulate_sign_tile!` :695-746 */
+            int R = d->num_secondary_rotations;
+            for (int lag = 0; lag < bs; lag++) {
+                int cp_col = off + lag;
+                for (size_t i = 0; i < (size_t)ndop * R; i++) S->stage[i] = 0;
+                o_fill_sign_stage_column(S->stage, S->tile, lag, cp_col, P, prn, S);
+                for (int s = 0; s < R; s++) {
+                    int base = s * spc;
+                    for (int w = 0; w < ndop; w++) {
+                        int sh = cp_col + (has_drift ? shifts[w] : 0);
+                        if (sh >= spc) sh -= spc;
+                        acc[(size_t)(base + sh) * ndop + w] += S->stage[(size_t)s * ndop + w];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Peak and noise: src/est_signal_noise_power.jl:109-243                     */
+/* ------------------------------------------------------------------------ */
+/* `_findmax_and_colsums!` :109-135 */
+static void o_findmax_and_colsums(const real* pb, int ndop, int ncols, real* col_sums, opeak* out)
+{
+    real peak = pb[0];
+    int pcol = 0;
+    for (int c = 0; c < ncols; c++) {
+        real cs = 0, cm = pb[(size_t)c * ndop];
+        for (int r = 0; r < ndop; r++) {
+            real v = pb[(size_t)c * ndop + r];
+            cs += v;
+            cm = cm > v ? cm : v;
+        }
+        col_sums[c] = cs;
+        if (cm > peak) { peak = cm; pcol = c; }
+    }
+    int prow = 0;
+    for (int r = 0; r < ndop; r++)
+        if (pb[(size_t)pcol * ndop + r] == peak) { prow = r; break; }
+    out->peak = peak;
+    out->row = prow;
+    out->col = pcol;
+}
+
+/* `_row_mean` :139-146 */
+static real o_row_mean(const real* pb, int ndop, int ncols, int row)
+{
+    real s = 0;
+    for (int c = 0; c < ncols; c++) s += pb[(size_t)c * ndop + row];
+    return s / (real)ncols;
+}
+
+/* `_global_mean_from_colsums` :150-170; peak_col is 0-based here. */
+static real o_global_mean_from_colsums(const real* col_sums, int ncols, int ndop, int peak_col,
+                                       int spchip)
+{
+    int pc1 = peak_col + 1;
+    int lo = pc1 - spchip > 1 ? pc1 - spchip : 1;
+    int hi = pc1 + spchip < ncols ? pc1 + spchip : ncols;
+    int nex = hi - lo + 1;
+    real total = 0, excl = 0;
+    for (int c = 1; c <= ncols; c++) {
+        real cs = col_sums[c - 1];
+        total += cs;
+        if (lo <= c && c <= hi) excl += cs;
+    }
+    return (total - excl) / (real)((int64_t)(ncols - nex) * ndop);
+}
+
+/* `_fmdbzp_column_to_tau` src/plot.jl:16-20 */
+int oracle_column_to_tau(int col, int nb, int bs)
+{
+    int block_row = col / bs;
+    int within = col % bs;
+    int m = (nb - block_row) % nb;
+    if (m < 0) m += nb;
+    return m * bs + within;
+}
+
+/* Julia `mod(x, y)` for Float64 (fmod, then shift into the sign of y). */
+static double jl_mod(double x, double y)
+{
+    double r = fmod(x, y);
+    if (r == 0.0) return copysign(0.0, y);
+    if ((r > 0) != (y > 0)) return r + y;
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Entry points                                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double peak;        /* peak power */
+    int32_t row, col;   /* 0-based sorted row, effective column */
+    double noise;       /* noise power */
+    double nbr[4];      /* code_left, code_right, dop_left, dop_right */
+    double doppler_hz;  /* result label (with interpolation when requested) */
+    double code_phase;  /* chips */
+    double cn0;         /* dB-Hz */
+    double ptn;         /* peak-to-noise ratio */
+    int32_t interp_code, interp_dop;
+} oresult;
+
+typedef struct {
+    double code_freq_hz;    /* get_code_frequency */
+    double code_length;     /* get_code_length */
+    double doppler_step_hz; /* step(plan.doppler_freqs) */
+    int32_t subsample_interpolation;
+    int32_t nthreads;
+} oopts;
+
+/* `_parabolic_interp` src/acquire.jl:9-13 on Float32 powers (the reference's
+ * operands are Float32, so its arithmetic is Float32). Returns 0 with
+ * *is_zero set when the denominator vanishes (the reference then returns the
+ * Float64 literal 0.0). */
+static double parabolic32(float l, float p, float r, int* is_zero)
+{
+    volatile float two_p = 2.0f * p;
+    volatile float a = two_p - l;
+    volatile float b = a - r;
+    volatile float denom = 2.0f * b;
+    if (denom == 0.0f) { *is_zero = 1; return 0.0; }
+    *is_zero = 0;
+    volatile float num = r - l;
+    volatile float q = num / denom;
+    return (double)q;
+}
+
+/* Result extraction shared by `_extract_result!` src/acquire.jl:544-671 and
+ * `_extract_result_streamed!` :681-781 (without the secondary-code phase,
+ * which the host evaluates on the raw signal). */
+static void o_extract(const oplan* P, const oopts* O, const double* doppler, real peak, int prow,
+                      int pcol, real noise, const real* surface, oresult* res)
+{
+    const fmdbzp_plan_desc* d = &P->d;
+    int ndop = d->num_doppler_bins, spc = d->samples_per_code;
+    double fs = d->sampling_freq_hz;
+    double code_period = O->code_length / O->code_freq_hz;
+    double tcoh = (double)d->num_coh * code_period;
+    res->peak = (double)peak;
+    res->row = prow;
+    res->col = pcol;
+    res->noise = (double)noise;
+    if (IS_F32) {
+        volatile float fnoise = (float)noise, fpeak = (float)peak;
+        volatile float sig = fpeak - fnoise;
+        volatile float num = sig + fnoise;
+        volatile float ptn = num / fnoise;
+        volatile float ratio = sig / fnoise;
+        res->ptn = (double)ptn;
+        res->cn0 = 10.0 * log10((double)ratio / tcoh / 1.0);
+    } else {
+        double sig = (double)peak - (double)noise;
+        res->ptn = (sig + (double)noise) / (double)noise;
+        res->cn0 = 10.0 * log10(sig / (double)noise / tcoh / 1.0);
+    }
+    int rot_block = pcol / spc;
+    int scrambled = pcol % spc;
+    int delay = oracle_column_to_tau(scrambled, d->num_blocks, d->block_size);
+    res->code_phase = jl_mod(-(double)delay * O->code_freq_hz / fs, O->code_length);
+    res->doppler_hz = doppler[prow];
+    int cl = ((scrambled - 1) % spc + spc) % spc + rot_block * spc;
+    int cr = (scrambled + 1) % spc + rot_block * spc;
+    int dl = prow == 0 ? ndop - 1 : prow - 1;
+    int dr = prow == ndop - 1 ? 0 : prow + 1;
+    res->nbr[0] = (double)surface[(size_t)cl * ndop + prow];
+    res->nbr[1] = (double)surface[(size_t)cr * ndop + prow];
+    res->nbr[2] = (double)surface[(size_t)pcol * ndop + dl];
+    res->nbr[3] = (double)surface[(size_t)pcol * ndop + dr];
+    res->interp_code = 0;
+    res->interp_dop = 0;
+    if (O->subsample_interpolation) {
+        float pl = (float)res->nbr[0], pr = (float)res->nbr[1];
+        float ql = (float)res->nbr[2], qr = (float)res->nbr[3];
+        float sq = sqrtf((float)noise);
+        float mx = pl > pr ? pl : pr;
+        if (mx > sq) {
+            int z;
+            double frac = parabolic32(pl, (float)peak, pr, &z);
+            res->interp_code = 1;
+            if (!z) {
+                /* Int + Float32 is Float32 in the reference (src/acquire.jl:605, :740) */
+                volatile float di = (float)delay + (float)frac;
+                volatile float ndi = -di;
+                res->code_phase = jl_mod((double)ndi * O->code_freq_hz / fs, O->code_length);
+            }
+        }
+        float mq = ql > qr ? ql : qr;
+        if (mq > sq) {
+            int z;
+            double frac = parabolic32(ql, (float)peak, qr, &z);
+            res->doppler_hz = doppler[prow] + frac * O->doppler_step_hz;
+            res->interp_dop = 1;
+        }
+    }
+}
+
+/*
+ * Full search of `nreq` PRNs: the body of `acquire!` (src/acquire.jl:54-215)
+ * up to and including result extraction, minus the secondary-code phase.
+ *
+ *  desc        plan descriptor (as built by the host from `plan_acquire`)
+ *  codes       nprn x spc int8 code samples (gen_code at phase 0)
+ *  patterns    nprn x (ncoh x Q) float sign patterns (column-major per PRN);
+ *              may be NULL on the simple path
+ *  shifts      nnc x ndop int32 code-drift shifts in [0, spc)
+ *  doppler     ndop doubles: plan.doppler_freqs in Hz
+ *  prn_index   nreq indices into codes
+ *  surfaces    NULL or nreq x (ndop x spc_eff) doubles, Julia column-major
+ */
+int oracle_acquire(const fmdbzp_plan_desc* desc, int nprn, const int8_t* codes,
+                   const float* patterns, const int32_t* shifts, const double* doppler,
+                   const void* signal, int eltype, int64_t nsamp, double if_hz,
+                   const int32_t* prn_index, int nreq, const oopts* opts, oresult* results,
+                   double* surfaces)
+{
+    oplan P;
+    memset(&P, 0, sizeof(P));
+    P.d = *desc;
+    const fmdbzp_plan_desc* d = &P.d;
+    int spc = d->samples_per_code, nb = d->num_blocks, bs = d->block_size, F = d->fft_size;
+    int ncoh = d->num_coh, ndop = d->num_doppler_bins, nnc = d->num_noncoherent;
+    int spc_eff = d->samples_per_code_eff, Q = d->num_patterns, R = d->num_secondary_rotations;
+    int64_t seglen = (int64_t)ncoh * spc;
+    if (nsamp / seglen < nnc) return -1;
+    P.nprn = nprn;
+    P.shifts = shifts;
+    if (offt_init(&P.PF, F) || offt_init(&P.PD, ndop)) return -2;
+
+    int maxn = F > ndop ? F : ndop;
+    size_t worksz = (size_t)4 * maxn;
+
+    /* plan-time tables */
+    P.prnconj = (cpx*)malloc(sizeof(cpx) * (size_t)nprn * nb * F);
+    {
+        cpx* work = (cpx*)malloc(sizeof(cpx) * worksz);
+        for (int p = 0; p < nprn; p++)
+            o_prn_conj_ffts(P.prnconj + (size_t)p * nb * F, codes + (size_t)p * spc, nb, bs, &P.PF, work);
+        free(work);
+    }
+    if (d->path != FMDBZP_PATH_SIMPLE) {
+        P.patterns = (real*)malloc(sizeof(real) * (size_t)nprn * ncoh * Q);
+        for (size_t i = 0; i < (size_t)nprn * ncoh * Q; i++) P.patterns[i] = (real)patterns[i];
+        if (d->path == FMDBZP_PATH_ROTATION) {
+            size_t tsz = (size_t)ndop * ncoh * Q;
+            P.tiled_re = (real*)malloc(sizeof(real) * nprn * tsz);
+            P.tiled_im = (real*)malloc(sizeof(real) * nprn * tsz);
+            for (int p = 0; p < nprn; p++)
+                o_tiled_phase(P.patterns + (size_t)p * ncoh * Q, ncoh, Q, ndop,
+                              P.tiled_re + p * tsz, P.tiled_im + p * tsz);
+        }
+    }
+
+    /* PRN-independent prologue: every segment's signal-block FFT cache
+     * (src/acquire.jl:121-134, :176-186) */
+    cpx* cache = (cpx*)malloc(sizeof(cpx) * (size_t)nnc * ncoh * nb * F);
+    {
+        cpx* seg = (cpx*)malloc(sizeof(cpx) * (size_t)seglen);
+        cpx* work = (cpx*)malloc(sizeof(cpx) * worksz);
+        for (int s = 0; s < nnc; s++) {
+            o_downconvert(seg, signal, eltype, (int64_t)s * seglen, seglen, if_hz, d->sampling_freq_hz);
+            o_signal_block_ffts(cache + (size_t)s * ncoh * nb * F, seg, spc, nb, bs, ncoh, &P.PF, work);
+        }
+        free(seg);
+        free(work);
+    }
+
+    int nthr = opts->nthreads > 0 ? opts->nthreads : 1;
+    int status = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthr)
+    for (int ri = 0; ri < nreq; ri++) {
+        int prn = prn_index[ri];
+        oscratch S;
+        S.work = (cpx*)malloc(sizeof(cpx) * worksz);
+        S.tile = (cpx*)malloc(sizeof(cpx) * (size_t)ndop * bs);
+        S.colbuf = (cpx*)malloc(sizeof(cpx) * (size_t)ndop);
+        int nsub = d->num_data_bits > ncoh ? d->num_data_bits : ncoh;
+        S.sub = (cpx*)malloc(sizeof(cpx) * (size_t)ndop * nsub);
+        S.cre = (real*)malloc(sizeof(real) * (size_t)ndop);
+        S.cim = (real*)malloc(sizeof(real) * (size_t)ndop);
+        S.stage = (real*)malloc(sizeof(real) * (size_t)ndop * R);
+        real* surface = (real*)calloc((size_t)ndop * spc_eff, sizeof(real));
+        real* col_sums = (real*)malloc(sizeof(real) * (size_t)spc_eff);
+        real* row_sums = (real*)malloc(sizeof(real) * (size_t)ndop);
+        opeak pk;
+        real noise;
+        if (nnc == 1) {
+            o_acquire_prn_streamed(&P, prn, cache, surface, row_sums, col_sums, &S, &pk);
+            if (d->noise_estimator == FMDBZP_NOISE_GLOBAL_MEAN) {
+                noise = o_global_mean_from_colsums(col_sums, spc_eff, ndop, pk.col, d->samples_per_chip);
+            } else {
+                int opp = (pk.row + ndop / 2) % ndop;
+                noise = row_sums[opp] / (real)spc_eff;
+            }
+        } else {
+            for (int s = 0; s < nnc; s++)
+                o_accumulate_prn_step(&P, prn, cache + (size_t)s * ncoh * nb * F, surface, s, &S);
+            o_findmax_and_colsums(surface, ndop, spc_eff, col_sums, &pk);
+            if (d->noise_estimator == FMDBZP_NOISE_GLOBAL_MEAN) {
+                noise = o_global_mean_from_colsums(col_sums, spc_eff, ndop, pk.col, d->samples_per_chip);
+            } else {
+                int opp = (pk.row + ndop / 2) % ndop;
+                noise = o_row_mean(surface, ndop, spc_eff, opp);
+            }
+        }
+        o_extract(&P, opts, doppler, pk.peak, pk.row, pk.col, noise, surface, &results[ri]);
+        if (surfaces) {
+            double* dst = surfaces + (size_t)ri * ndop * spc_eff;
+            for (size_t i = 0; i < (size_t)ndop * spc_eff; i++) dst[i] = (double)surface[i];
+        }
+        free(S.work); free(S.tile); free(S.colbuf); free(S.sub); free(S.cre); free(S.cim);
+        free(S.stage); free(surface); free(col_sums); free(row_sums);
+    }
+
+    free(cache);
+    free(P.prnconj);
+    free(P.patterns);
+    free(P.tiled_re);
+    free(P.tiled_im);
+    offt_free(&P.PF);
+    offt_free(&P.PD);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Secondary-code phase: `_estimate_secondary_code_phase` src/acquire.jl:503-538 */
+/* ------------------------------------------------------------------------ */
+/* code: spc int8 samples of gen_code(spc, system, prn, fs, f_code, code_phase);
+ * sec: L int8 secondary chips of this PRN. Returns the best rotation. */
+int oracle_secondary_phase(const void* signal, int eltype, int spc, int L, double fs, double if_hz,
+                           double doppler_hz, const int8_t* code, const int8_t* sec)
+{
+    double phase_step = (-2.0 * M_PI * (if_hz + doppler_hz)) / fs;
+    float dre = (float)cos(phase_step), dim = (float)sin(phase_step);
+    float zr[64], zi[64];
+    if (L > 64) return -1;
+    for (int k = 0; k < L; k++) {
+        int64_t base = (int64_t)k * spc;
+        double a = phase_step * (double)base;
+        float cr = (float)cos(a), ci = (float)sin(a);
+        float ar = 0.0f, ai = 0.0f;
+        for (int n = 0; n < spc; n++) {
+            float sr, si;
+            load_sample(signal, eltype, base + n, &sr, &si);
+            /* (sig * carr) * code, ComplexF32 products, left to right */
+            volatile float t1 = sr * cr, t2 = si * ci, t3 = sr * ci, t4 = si * cr;
+            volatile float pr = t1 - t2, pi_ = t3 + t4;
+            float c = (float)code[n];
+            volatile float u1 = pr * c, u2 = pi_ * 0.0f, u3 = pr * 0.0f, u4 = pi_ * c;
+            volatile float qr = u1 - u2, qi = u3 + u4;
+            ar += qr;
+            ai += qi;
+            /* carr *= dphi */
+            volatile float v1 = cr * dre, v2 = ci * dim, v3 = cr * dim, v4 = ci * dre;
+            cr = v1 - v2;
+            ci = v3 + v4;
+        }
+        zr[k] = ar;
+        zi[k] = ai;
+    }
+    int best_r = 0;
+    float best = -1.0f;
+    for (int r = 0; r < L; r++) {
+        float sr = 0.0f, si = 0.0f;
+        for (int k = 0; k < L; k++) {
+            float sv = (float)sec[(k + r) % L];
+            volatile float a = sv * zr[k], b = sv * zi[k];
+            sr += a;
+            si += b;
+        }
+        float mag = sr * sr + si * si;
+        if (mag > best) { best = mag; best_r = r; }
+    }
+    return best_r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CFAR: `cfar_threshold` src/cfar.jl:64-85                                  */
+/* ------------------------------------------------------------------------ */
+/* log of the regularised upper incomplete gamma Q(a, x) for integer a >= 1:
+ * Q(a, x) = exp(-x) sum_{k<a} x^k / k!  (closed form for integer shape). */
+static double log_q_int(int a, double x)
+{
+    /* sum in log space, largest term first for accuracy */
+    double lt = 0.0, lmax = 0.0;
+    double terms[4096];
+    if (a > 4096) return NAN;
+    for (int k = 0; k < a; k++) {
+        if (k == 0) lt = 0.0;
+        else lt += log(x) - log((double)k);
+        terms[k] = lt;
+        if (k == 0 || lt > lmax) lmax = lt;
+    }
+    double s = 0.0;
+    for (int k = 0; k < a; k++) s += exp(terms[k] - lmax);
+    return -x + lmax + log(s);
+}
+
+double oracle_cfar_threshold(double pfa, double num_cells, int M)
+{
+    if (!(pfa > 0 && pfa < 1) || !(num_cells > 0) || M <= 0) return NAN;
+    double p_cell = 1.0 - pow(1.0 - pfa, 1.0 / num_cells);
+    double lq = log(p_cell);
+    /* bracket and bisect/Newton on f(x) = log Q(M, x) - log p (decreasing in x) */
+    double lo = 0.0, hi = 1.0;
+    while (log_q_int(M, hi) > lq) hi *= 2.0;
+    double x = 0.5 * (lo + hi);
+    for (int it = 0; it < 200; it++) {
+        double f = log_q_int(M, x) - lq;
+        if (f > 0) lo = x; else hi = x;
+        /* Newton step: d/dx log Q = -x^(M-1) e^-x / ((M-1)! Q) */
+        double lpdf = (M - 1) * log(x) - x - lgamma((double)M);
+        double dfdx = -exp(lpdf - log_q_int(M, x));
+        double xn = x - f / dfdx;
+        if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+        if (fabs(xn - x) <= 1e-15 * fabs(x)) { x = xn; break; }
+        x = xn;
+    }
+    return x / (double)M;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Small entry points used by the oracle's own pinning tests                 */
+/* ------------------------------------------------------------------------ */
+/* In-place FFT of n complex values given as doubles (re, im interleaved). */
+int oracle_fft(int n, double* data, int inverse)
+{
+    offt P;
+    if (offt_init(&P, n)) return -1;
+    cpx* buf = (cpx*)malloc(sizeof(cpx) * (size_t)n);
+    cpx* work = (cpx*)malloc(sizeof(cpx) * (size_t)n * 4);
+    for (int i = 0; i < n; i++) { buf[i].re = (real)data[2 * i]; buf[i].im = (real)data[2 * i + 1]; }
+    offt_exec(&P, buf, work, inverse);
+    for (int i = 0; i < n; i++) { data[2 * i] = (double)buf[i].re; data[2 * i + 1] = (double)buf[i].im; }
+    free(buf);
+    free(work);
+    offt_free(&P);
+    return 0;
+}
+
+/* `est_signal_noise_power` src/est_signal_noise_power.jl:211-243 on a
+ * (ndop, ncols) column-major matrix given as doubles. out: signal, noise,
+ * code bin (1-based), doppler bin (1-based) as doubles. */
+int oracle_est_signal_noise_power(const double* pb_in, int ndop, int ncols, double fs, double fc,
+                                  int estimator, double* out)
+{
+    real* pb = (real*)malloc(sizeof(real) * (size_t)ndop * ncols);
+    real* cs = (real*)malloc(sizeof(real) * (size_t)ncols);
+    for (size_t i = 0; i < (size_t)ndop * ncols; i++) pb[i] = (real)pb_in[i];
+    opeak pk;
+    o_findmax_and_colsums(pb, ndop, ncols, cs, &pk);
+    real noise;
+    if (estimator == FMDBZP_NOISE_GLOBAL_MEAN) {
+        int spchip = (int)floor(fs / fc);
+        noise = o_global_mean_from_colsums(cs, ncols, ndop, pk.col, spchip);
+    } else {
+        int opp = (pk.row + ndop / 2) % ndop;
+        noise = o_row_mean(pb, ndop, ncols, opp);
+    }
+    out[0] = (double)(pk.peak - noise);
+    out[1] = (double)noise;
+    out[2] = (double)(pk.col + 1);
+    out[3] = (double)(pk.row + 1);
+    free(pb);
+    free(cs);
+    return 0;
+}
+
+/* Signal-block FFT cache of one segment (for stage-level parity tests):
+ * out is (ncoh*nb) x F complex as interleaved doubles. */
+int oracle_signal_block_ffts(const fmdbzp_plan_desc* d, const void* signal, int eltype, int64_t seg0,
+                             double if_hz, double* out)
+{
+    int spc = d->samples_per_code, nb = d->num_blocks, bs = d->block_size, F = d->fft_size;
+    int ncoh = d->num_coh;
+    int64_t seglen = (int64_t)ncoh * spc;
+    offt PF;
+    if (offt_init(&PF, F)) return -1;
+    cpx* seg = (cpx*)malloc(sizeof(cpx) * (size_t)seglen);
+    cpx* work = (cpx*)malloc(sizeof(cpx) * (size_t)4 * F);
+    cpx* cache = (cpx*)malloc(sizeof(cpx) * (size_t)ncoh * nb * F);
+    o_downconvert(seg, signal, eltype, seg0, seglen, if_hz, d->sampling_freq_hz);
+    o_signal_block_ffts(cache, seg, spc, nb, bs, ncoh, &PF, work);
+    for (size_t i = 0; i < (size_t)ncoh * nb * F; i++) {
+        out[2 * i] = (double)cache[i].re;
+        out[2 * i + 1] = (double)cache[i].im;
+    }
+    free(seg); free(work); free(cache);
+    offt_free(&PF);
+    return 0;
+}
+
+/* Full coherent integration matrix of one PRN for one segment
+ * (`_build_coherent_integration_matrix!` src/coherent_integration.jl:90-113):
+ * out is (ndop, spc) column-major complex as interleaved doubles. */
+int oracle_coherent_matrix(const fmdbzp_plan_desc* d, const int8_t* code, const void* signal,
+                           int eltype, int64_t seg0, double if_hz, double* out)
+{
+    int spc = d->samples_per_code, nb = d->num_blocks, bs = d->block_size, F = d->fft_size;
+    int ncoh = d->num_coh, ndop = d->num_doppler_bins;
+    int64_t seglen = (int64_t)ncoh * spc;
+    offt PF;
+    if (offt_init(&PF, F)) return -1;
+    cpx* seg = (cpx*)malloc(sizeof(cpx) * (size_t)seglen);
+    cpx* work = (cpx*)malloc(sizeof(cpx) * (size_t)4 * F);
+    cpx* cache = (cpx*)malloc(sizeof(cpx) * (size_t)ncoh * nb * F);
+    cpx* pc = (cpx*)malloc(sizeof(cpx) * (size_t)nb * F);
+    cpx* tile = (cpx*)malloc(sizeof(cpx) * (size_t)ndop * bs);
+    o_downconvert(seg, signal, eltype, seg0, seglen, if_hz, d->sampling_freq_hz);
+    o_signal_block_ffts(cache, seg, spc, nb, bs, ncoh, &PF, work);
+    o_prn_conj_ffts(pc, code, nb, bs, &PF, work);
+    for (int r = 0; r < nb; r++) {
+        o_build_tile(tile, cache, pc, r, nb, bs, ncoh, &PF, work);
+        for (int lag = 0; lag < bs; lag++)
+            for (int k = 0; k < ndop; k++) {
+                size_t o = (size_t)(r * bs + lag) * ndop + k;
+                out[2 * o] = (double)tile[(size_t)lag * ndop + k].re;
+                out[2 * o + 1] = (double)tile[(size_t)lag * ndop + k].im;
+            }
+    }
+    free(seg); free(work); free(cache); free(pc); free(tile);
+    offt_free(&PF);
+    return 0;
+}
+
+/* `_downconvert!` alone, for the Float32 IF-phase check: out is seglen
+ * complex as interleaved doubles. */
+int oracle_downconvert(const void* signal, int eltype, int64_t seg0, int64_t seglen, double if_hz,
+                       double fs, double* out)
+{
+    cpx* seg = (cpx*)malloc(sizeof(cpx) * (size_t)seglen);
+    o_downconvert(seg, signal, eltype, seg0, seglen, if_hz, fs);
+    for (int64_t i = 0; i < seglen; i++) {
+        out[2 * i] = (double)seg[i].re;
+        out[2 * i + 1] = (double)seg[i].im;
+    }
+    free(seg);
+    return 0;
+}
+
+int oracle_real_bytes(void) { return (int)sizeof(real); }

@@ -1,0 +1,244 @@
+"""ctypes wrapper of the CPU restatement (fmdbzp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests, by the smoke check and by the
+benchmark's CPU-baseline leg, always as the checker or the timed baseline,
+never as a product code path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+
+class OResult(ctypes.Structure):
+    _fields_ = [
+        ("peak", ctypes.c_double),
+        ("row", ctypes.c_int32),
+        ("col", ctypes.c_int32),
+        ("noise", ctypes.c_double),
+        ("nbr", ctypes.c_double * 4),
+        ("doppler_hz", ctypes.c_double),
+        ("code_phase", ctypes.c_double),
+        ("cn0", ctypes.c_double),
+        ("ptn", ctypes.c_double),
+        ("interp_code", ctypes.c_int32),
+        ("interp_dop", ctypes.c_int32),
+    ]
+
+
+class OOpts(ctypes.Structure):
+    _fields_ = [
+        ("code_freq_hz", ctypes.c_double),
+        ("code_length", ctypes.c_double),
+        ("doppler_step_hz", ctypes.c_double),
+        ("subsample_interpolation", ctypes.c_int32),
+        ("nthreads", ctypes.c_int32),
+    ]
+
+
+def build():
+    """Compile liboracle32.so / liboracle64.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib(bits: int = 32) -> ctypes.CDLL:
+    if bits in _LIBS:
+        return _LIBS[bits]
+    path = os.path.join(_HERE, "_build", f"liboracle{bits}.so")
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    L.oracle_acquire.argtypes = [P, ctypes.c_int, P, P, P, P, P, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                                 P, ctypes.c_int, ctypes.POINTER(OOpts), P, P]
+    L.oracle_acquire.restype = ctypes.c_int
+    L.oracle_fft.argtypes = [ctypes.c_int, P, ctypes.c_int]
+    L.oracle_fft.restype = ctypes.c_int
+    L.oracle_est_signal_noise_power.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int, P]
+    L.oracle_est_signal_noise_power.restype = ctypes.c_int
+    L.oracle_cfar_threshold.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.oracle_cfar_threshold.restype = ctypes.c_double
+    L.oracle_column_to_tau.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.oracle_column_to_tau.restype = ctypes.c_int
+    L.oracle_sign_patterns.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    L.oracle_sign_patterns.restype = None
+    L.oracle_signal_block_ffts.argtypes = [P, P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, P]
+    L.oracle_signal_block_ffts.restype = ctypes.c_int
+    L.oracle_coherent_matrix.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int64, ctypes.c_double, P]
+    L.oracle_coherent_matrix.restype = ctypes.c_int
+    L.oracle_downconvert.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                     ctypes.c_double, P]
+    L.oracle_downconvert.restype = ctypes.c_int
+    L.oracle_secondary_phase.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, P, P]
+    L.oracle_secondary_phase.restype = ctypes.c_int
+    L.oracle_real_bytes.restype = ctypes.c_int
+    _LIBS[bits] = L
+    return L
+
+
+ELTYPE = {"cf32": 0, "cf64": 1, "i16": 2, "ci8": 3, "ci16": 4, "f32": 5, "f64": 6}
+
+
+def _eltype(a: np.ndarray):
+    if a.dtype == np.complex64:
+        return ELTYPE["cf32"], a.shape[0]
+    if a.dtype == np.complex128:
+        return ELTYPE["cf64"], a.shape[0]
+    if a.dtype == np.int16 and a.ndim == 1:
+        return ELTYPE["i16"], a.shape[0]
+    if a.dtype == np.int16 and a.ndim == 2:
+        return ELTYPE["ci16"], a.shape[0]
+    if a.dtype == np.int8 and a.ndim == 2:
+        return ELTYPE["ci8"], a.shape[0]
+    if a.dtype == np.float32:
+        return ELTYPE["f32"], a.shape[0]
+    if a.dtype == np.float64:
+        return ELTYPE["f64"], a.shape[0]
+    raise TypeError(f"unsupported signal dtype {a.dtype}")
+
+
+def desc_struct(plan):
+    """fmdbzp_plan_desc of a host plan (geometry only)."""
+    import sys
+
+    pkg = os.path.join(os.path.dirname(_HERE), "acquisition.jl_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)
+    from acquisition._native import make_desc
+    from acquisition.plan import plan_desc_fields
+
+    return make_desc(plan_desc_fields(plan))
+
+
+def acquire(plan, signal, prn_index, *, if_hz=0.0, subsample_interpolation=False, surfaces=False,
+            nthreads=None, bits=32, codes=None):
+    """Run the restatement for plan PRN indices `prn_index`. Returns (results,
+    surfaces) with results a list of dicts and surfaces an array
+    (nreq, ndop, spc_eff) in Julia column-major semantics (or None)."""
+    import sys
+
+    pkg = os.path.join(os.path.dirname(_HERE), "acquisition.jl_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)
+    from acquisition.plan import plan_code_samples
+
+    L = lib(bits)
+    d = desc_struct(plan)
+    if codes is None:
+        codes = plan_code_samples(plan)
+    codes = np.ascontiguousarray(codes, dtype=np.int8)
+    if plan.path != 0:
+        pats = np.stack([np.ascontiguousarray(plan.sign_patterns_by_prn[p].T) for p in plan.avail_prns])
+        pats = np.ascontiguousarray(pats, dtype=np.float32)
+    else:
+        pats = np.ones(1, dtype=np.float32)
+    shifts = np.ascontiguousarray(plan.code_drift_shifts, dtype=np.int32)
+    dop = np.ascontiguousarray(plan.doppler_freqs, dtype=np.float64)
+    sig = np.ascontiguousarray(signal)
+    et, n = _eltype(sig)
+    idx = np.ascontiguousarray(prn_index, dtype=np.int32)
+    nreq = len(idx)
+    res = (OResult * nreq)()
+    opts = OOpts()
+    opts.code_freq_hz = plan.system.get_code_frequency()
+    opts.code_length = float(plan.system.get_code_length())
+    opts.doppler_step_hz = plan.doppler_spacing_hz
+    opts.subsample_interpolation = 1 if subsample_interpolation else 0
+    opts.nthreads = nthreads or min(os.cpu_count() or 1, 16)
+    surf = None
+    if surfaces:
+        surf = np.zeros((nreq, plan.samples_per_code_eff, plan.num_doppler_bins), dtype=np.float64)
+    rc = L.oracle_acquire(ctypes.byref(d), len(plan.avail_prns), codes.ctypes.data, pats.ctypes.data,
+                          shifts.ctypes.data, dop.ctypes.data, sig.ctypes.data, et, n, float(if_hz),
+                          idx.ctypes.data, nreq, ctypes.byref(opts), ctypes.cast(res, ctypes.c_void_p),
+                          surf.ctypes.data if surf is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"oracle_acquire failed: {rc}")
+    out = []
+    for r in res:
+        out.append(dict(peak=r.peak, row=r.row, col=r.col, noise=r.noise, nbr=list(r.nbr),
+                        doppler_hz=r.doppler_hz, code_phase=r.code_phase, cn0=r.cn0, ptn=r.ptn,
+                        interp_code=r.interp_code, interp_dop=r.interp_dop))
+    if surf is not None:
+        # memory is [col][row]: return (nreq, ndop, spc_eff) views
+        surf = np.transpose(surf, (0, 2, 1))
+    return out, surf
+
+
+def fft(x, inverse=False, bits=32):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).copy()
+    buf = a.view(np.float64)
+    lib(bits).oracle_fft(len(a), buf.ctypes.data, 1 if inverse else 0)
+    return buf.view(np.complex128)
+
+
+def est_signal_noise_power(power_bins, fs, fc, estimator=0, bits=32):
+    pb = np.asfortranarray(np.asarray(power_bins, dtype=np.float64))
+    out = np.zeros(4)
+    lib(bits).oracle_est_signal_noise_power(pb.ctypes.data, pb.shape[0], pb.shape[1], float(fs), float(fc),
+                                            int(estimator), out.ctypes.data)
+    return out[0], out[1], int(out[2]), int(out[3])
+
+
+def cfar_threshold(pfa, num_cells, M=1, bits=64):
+    return lib(bits).oracle_cfar_threshold(float(pfa), float(num_cells), int(M))
+
+
+def column_to_tau(col, nb, bs):
+    return lib(32).oracle_column_to_tau(int(col), int(nb), int(bs))
+
+
+def sign_patterns(sec, ndb, R, ncoh, use_sec):
+    s = np.ascontiguousarray(np.asarray(sec if sec is not None else [1], dtype=np.int8))
+    ndc = 1 << (ndb - 1)
+    nsec = R if (use_sec and R > 1) else 1
+    out = np.zeros(ncoh * ndc * nsec, dtype=np.float32)
+    lib(32).oracle_sign_patterns(s.ctypes.data, len(s), ndb, R, ncoh, 1 if use_sec else 0, out.ctypes.data)
+    return out.reshape(ndc * nsec, ncoh).T
+
+
+def signal_block_ffts(plan, signal, seg0=0, if_hz=0.0, bits=32):
+    d = desc_struct(plan)
+    sig = np.ascontiguousarray(signal)
+    et, _ = _eltype(sig)
+    nrow = plan.num_coherently_integrated_code_periods * plan.num_blocks
+    out = np.zeros(nrow * plan.fft_size * 2)
+    lib(bits).oracle_signal_block_ffts(ctypes.byref(d), sig.ctypes.data, et, int(seg0), float(if_hz), out.ctypes.data)
+    return out.view(np.complex128).reshape(nrow, plan.fft_size)
+
+
+def coherent_matrix(plan, code, signal, seg0=0, if_hz=0.0, bits=32):
+    d = desc_struct(plan)
+    sig = np.ascontiguousarray(signal)
+    et, _ = _eltype(sig)
+    c = np.ascontiguousarray(code, dtype=np.int8)
+    out = np.zeros(plan.num_doppler_bins * plan.samples_per_code * 2)
+    lib(bits).oracle_coherent_matrix(ctypes.byref(d), c.ctypes.data, sig.ctypes.data, et, int(seg0), float(if_hz),
+                                     out.ctypes.data)
+    # memory [col][row]
+    return out.view(np.complex128).reshape(plan.samples_per_code, plan.num_doppler_bins).T
+
+
+def downconvert(signal, seg0, seglen, if_hz, fs, bits=32):
+    sig = np.ascontiguousarray(signal)
+    et, _ = _eltype(sig)
+    out = np.zeros(seglen * 2)
+    lib(bits).oracle_downconvert(sig.ctypes.data, et, int(seg0), int(seglen), float(if_hz), float(fs), out.ctypes.data)
+    return out.view(np.complex128)
+
+
+def secondary_phase(signal, spc, L, fs, if_hz, doppler_hz, code, sec):
+    sig = np.ascontiguousarray(signal)
+    et, _ = _eltype(sig)
+    c = np.ascontiguousarray(code, dtype=np.int8)
+    s = np.ascontiguousarray(sec, dtype=np.int8)
+    return lib(32).oracle_secondary_phase(sig.ctypes.data, et, int(spc), int(L), float(fs), float(if_hz),
+                                          float(doppler_hz), c.ctypes.data, s.ctypes.data)
