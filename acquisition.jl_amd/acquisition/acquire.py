@@ -117,8 +117,15 @@ class _SignalView:
     """Pointer, sample format and length of a host or device signal."""
 
     def __init__(self, signal):
+        from .device import DeviceArray
+
         self.obj = signal
-        if _is_torch(signal):
+        if isinstance(signal, DeviceArray):
+            self.kind, self.n = _numpy_eltype(np.empty((0,) + tuple(signal.shape[1:]), dtype=signal.dtype))
+            self.n = signal.shape[0]
+            self.device = True
+            self.ptr = signal.ptr
+        elif _is_torch(signal):
             import torch
 
             t = signal
@@ -150,7 +157,12 @@ class _SignalView:
 
     def complex64_prefix(self, n: int) -> np.ndarray:
         """ComplexF32 of the first n samples on the host."""
-        x = self.obj[:n]
+        from .device import DeviceArray
+
+        if isinstance(self.obj, DeviceArray):
+            x = self.obj.to_host(n)
+        else:
+            x = self.obj[:n]
         if _is_torch(x):
             x = x.cpu().numpy()
         if self.kind in ("ci8", "ci16"):
@@ -267,7 +279,9 @@ def acquire_(plan: AcquisitionPlan, signal, prns: Union[int, Sequence[int]], *, 
 
     `signal` is a 1-D numpy array (complex64, complex128, int16 real,
     float32/float64 real), an (n, 2) int8/int16 array of complex integer
-    samples, or the same as a torch tensor (a CUDA tensor is read in place).
+    samples, the same in device memory (`device.DeviceArray`, read in place),
+    or a torch tensor (a CUDA tensor is read in place; torch must then be
+    imported before this package so one HIP runtime serves both).
     Returns one `AcquisitionResults` per PRN (a single result for an int).
     """
     if isinstance(prns, (int, np.integer)):
@@ -301,8 +315,14 @@ def acquire_(plan: AcquisitionPlan, signal, prns: Union[int, Sequence[int]], *, 
     stats = plan.native.acquire(sv.ptr, _native.ELTYPE[sv.kind], sv.n, float(interm_freq), idx, flags, surfaces)
     results = []
     for j, p in enumerate(prns):
-        results.append(_extract(plan, stats[j], p, sv, float(interm_freq), subsample_interpolation,
-                                bufs[j] if bufs is not None else None))
+        r = _extract(plan, stats[j], p, sv, float(interm_freq), subsample_interpolation,
+                     bufs[j] if bufs is not None else None)
+        st = stats[j]
+        r.engine_stats = dict(peak=float(st.peak_power), row=int(st.peak_row), col=int(st.peak_col),
+                              noise=float(st.noise_power), nbr=[float(st.code_left), float(st.code_right),
+                                                                float(st.dop_left), float(st.dop_right)],
+                              noise_sum=float(st.noise_sum), noise_cells=int(st.noise_cells))
+        results.append(r)
     plan.acq_results_buf = results
     return results
 

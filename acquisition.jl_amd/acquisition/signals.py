@@ -1,0 +1,161 @@
+"""Seeded synthetic GNSS signals for tests and measurement.
+
+The reference's `generate_test_signal` (src/generate_test_signal.jl:32-82)
+draws its noise from Julia's RNG, which cannot be reproduced outside Julia;
+these generators use the counter-based Philox stream of `rng.py` instead, so
+every run (CPU or GPU, any process) sees identical sample arrays for a seed.
+
+`generate_test_signal` keeps the reference's signal model: code (with code
+Doppler) x carrier x 10^(CN0/20) plus complex noise scaled to
+10^(10 log10(fs)/20), or to unit power with `unit_noise_power`.
+The workload generators build the benchmark scenes:
+  l1ca_5mhz_scene      GPS L1 C/A, 5 MHz complex baseband, 1 ms, one PRN
+  l1ca_if_int16_scene  GPS L1 C/A, 16.368 MHz real int16 at IF 4.092 MHz,
+                       80 ms, 8 PRNs with 50 bps data
+  e1like_scene         E1-like BOC(1,1) pilots with a 25-chip secondary code,
+                       25 MHz complex baseband, 4 x 100 ms
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from .rng import PhiloxStream
+from .systems import GNSSSystem, GPSL1CA, E1Like
+
+
+def _noise_c(stream: PhiloxStream, n: int) -> np.ndarray:
+    g = stream.normal(2 * n)
+    return (g[0::2] + 1j * g[1::2]).astype(np.complex128)
+
+
+def generate_test_signal(system: GNSSSystem, prn: int, *, seed: int = 2345, num_samples: int = 120000,
+                         doppler: float = 1234.0, code_phase: float = 110.613261,
+                         sampling_freq: float = 15e6 - 1, interm_freq: float = 243.0, CN0: float = 45,
+                         phase_offset: float = math.pi / 8, unit_noise_power: bool = False,
+                         noise: bool = True) -> dict:
+    """Signal model of the reference's `generate_test_signal` with seeded Philox
+    noise: complex normal of unit total variance (as Julia's randn(ComplexF64)),
+    scaled as the reference scales it. `noise=False` gives the clean signal."""
+    code_freq = system.get_code_frequency() + doppler * system.get_code_center_frequency_ratio()
+    code = system.gen_code(num_samples, prn, sampling_freq, code_freq, code_phase).astype(np.float64)
+    t = np.arange(num_samples, dtype=np.float64)
+    carrier = np.exp(1j * (2 * math.pi * t * (interm_freq + doppler) / sampling_freq + phase_offset))
+    if unit_noise_power:
+        noise_power = 0.0
+        signal_power = CN0 - 10 * math.log10(sampling_freq)
+    else:
+        noise_power = 10 * math.log10(sampling_freq)
+        signal_power = CN0
+    sig = (carrier * code) * 10 ** (signal_power / 20)
+    if noise:
+        # Julia's randn(ComplexF64) has unit total variance: real/imag N(0, 1/2)
+        n = _noise_c(PhiloxStream(seed, 1), num_samples) / math.sqrt(2.0)
+        sig = sig + n * 10 ** (noise_power / 20)
+    return dict(signal=sig, code=code, carrier=carrier, num_samples=num_samples, doppler=doppler,
+                code_phase=code_phase, sampling_freq=sampling_freq, interm_freq=interm_freq, CN0=CN0, prn=prn)
+
+
+@dataclass
+class Scene:
+    signal: np.ndarray
+    sampling_freq: float
+    interm_freq: float
+    system: GNSSSystem
+    prns: List[int]
+    present: List[int]
+    truth: dict = field(default_factory=dict)
+
+
+def l1ca_5mhz_scene(seed: int = 0x5EED0001, cn0: float = 45.0) -> Scene:
+    """5000 complex samples (1 ms at 5 MHz, IF 0), unit-variance complex AWGN,
+    one L1 C/A PRN from 1..32 at `cn0` dB-Hz, code phase U[0,1023) chips,
+    Doppler U[-5,5] kHz, carrier phase U[0, 2 pi)."""
+    sysm = GPSL1CA()
+    fs = 5e6
+    n = 5000
+    st = PhiloxStream(seed, 0)
+    prn = int(st.integers(1, 1, 33)[0])
+    cp, fd, ph = st.uniform(3) * np.array([1023.0, 1.0, 1.0])
+    fd = -5000.0 + 10000.0 * fd
+    ph = 2 * math.pi * ph
+    amp = math.sqrt(10 ** (cn0 / 10) / fs)
+    code_freq = sysm.code_frequency + fd * sysm.get_code_center_frequency_ratio()
+    code = sysm.gen_code(n, prn, fs, code_freq, cp).astype(np.float64)
+    t = np.arange(n, dtype=np.float64)
+    carr = np.exp(1j * (2 * math.pi * fd * t / fs + ph))
+    noise = _noise_c(PhiloxStream(seed, 1), n) / math.sqrt(2.0)
+    sig = (amp * code * carr + noise).astype(np.complex64)
+    return Scene(sig, fs, 0.0, sysm, list(range(1, 33)), [prn],
+                 dict(prn=prn, code_phase=cp, doppler=fd, cn0=cn0))
+
+
+def l1ca_if_int16_scene(seed: int = 0x5EED0002, n_ms: int = 80, fs: float = 16.368e6, if_hz: float = 4.092e6,
+                        npresent: int = 8, sigma: float = 1000.0, cn0_range=(30.0, 40.0)) -> Scene:
+    """Real int16 IF samples: `npresent` of PRNs 1..32 at CN0 U[cn0_range],
+    Doppler U[-5,5] kHz, code phase U[0,1023), +-1 data at 50 bps with a
+    random bit-edge offset in whole ms, Gaussian noise sigma LSB.
+    C/N0 = A^2 fs / (4 sigma^2) for a real carrier A cos(.)."""
+    sysm = GPSL1CA()
+    n = int(round(fs * n_ms / 1000.0))
+    st = PhiloxStream(seed, 0)
+    perm = np.argsort(st.uniform(32))
+    present = sorted(int(p) + 1 for p in perm[:npresent])
+    t = np.arange(n, dtype=np.float64)
+    x = PhiloxStream(seed, 1).normal(n) * sigma
+    truth = {}
+    for i, prn in enumerate(present):
+        u = PhiloxStream(seed, 100 + prn).uniform(5)
+        cn0 = cn0_range[0] + (cn0_range[1] - cn0_range[0]) * u[0]
+        fd = -5000.0 + 10000.0 * u[1]
+        cp = 1023.0 * u[2]
+        ph = 2 * math.pi * u[3]
+        edge = int(u[4] * 20)
+        amp = math.sqrt(4 * sigma ** 2 * 10 ** (cn0 / 10) / fs)
+        code_freq = sysm.code_frequency + fd * sysm.get_code_center_frequency_ratio()
+        code = sysm.gen_code(n, prn, fs, code_freq, cp).astype(np.float64)
+        nbits = n_ms // 20 + 2
+        bits = PhiloxStream(seed, 200 + prn).signs(nbits).astype(np.float64)
+        ms = np.floor(t * 1000.0 / fs).astype(np.int64)
+        data = bits[(ms + edge) // 20]
+        x += amp * data * code * np.cos(2 * math.pi * (if_hz + fd) * t / fs + ph)
+        truth[prn] = dict(cn0=cn0, doppler=fd, code_phase=cp, edge_ms=edge)
+    sig = np.clip(np.rint(x), -32768, 32767).astype(np.int16)
+    return Scene(sig, fs, if_hz, sysm, list(range(1, 33)), present, truth)
+
+
+def e1like_scene(seed: int = 0x5EED0004, n_periods: int = 100, nnc: int = 4, fs: float = 25e6,
+                 nprn: int = 36, npresent: int = 6, cn0_range=(28.0, 38.0), system=None) -> Scene:
+    """E1-like pilots: complex baseband at fs, `nnc` segments of `n_periods`
+    4 ms primary periods; `npresent` of `nprn` PRNs with secondary phase
+    U{0..24}, Doppler U[-5,5] kHz, CN0 U[cn0_range]; unit-variance noise."""
+    sysm = system or E1Like()
+    spc = math.ceil(sysm.code_length / sysm.code_frequency * fs)
+    n = spc * n_periods * nnc
+    st = PhiloxStream(seed, 0)
+    perm = np.argsort(st.uniform(nprn))
+    present = sorted(int(p) + 1 for p in perm[:npresent])
+    t = np.arange(n, dtype=np.float64)
+    x = _noise_c(PhiloxStream(seed, 1), n) / math.sqrt(2.0)
+    L = sysm.get_secondary_code_length()
+    truth = {}
+    for prn in present:
+        u = PhiloxStream(seed, 100 + prn).uniform(5)
+        cn0 = cn0_range[0] + (cn0_range[1] - cn0_range[0]) * u[0]
+        fd = -5000.0 + 10000.0 * u[1]
+        cp = sysm.code_length * u[2]
+        ph = 2 * math.pi * u[3]
+        sec0 = int(u[4] * L)
+        amp = math.sqrt(10 ** (cn0 / 10) / fs)
+        code_freq = sysm.code_frequency + fd * sysm.get_code_center_frequency_ratio()
+        pos = cp + (t * code_freq) / fs
+        code = sysm.gen_code(n, prn, fs, code_freq, cp).astype(np.float64)
+        per = np.floor(pos / sysm.code_length).astype(np.int64)
+        sec = sysm.secondary_code(prn).astype(np.float64)
+        code = code * sec[(per + sec0) % L]
+        x += amp * code * np.exp(1j * (2 * math.pi * fd * t / fs + ph))
+        truth[prn] = dict(cn0=cn0, doppler=fd, code_phase=cp, secondary_phase=sec0)
+    return Scene(x.astype(np.complex64), fs, 0.0, sysm, list(range(1, nprn + 1)), present, truth)

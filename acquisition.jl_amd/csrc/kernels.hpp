@@ -474,6 +474,11 @@ k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
             row = (int)(key % (uint64_t)ndop);
             col = (int)(key / (uint64_t)ndop);
         }
+        // no comparable cell (all NaN input): report cell (0, 0), never index out of range
+        if (key == ~0ull || row < 0 || row >= ndop || col < 0 || col >= spc_eff) {
+            row = 0;
+            col = 0;
+        }
         s_row = row;
         s_col = col;
     }

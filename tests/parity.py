@@ -1,0 +1,88 @@
+"""Comparison helpers shared by the GPU parity tests.
+
+Tolerances (FP32 path vs the FP32 oracle; both are FP32 FFT pipelines with
+different transform algorithms, so cell values differ by FP32 rounding):
+  SURF_GLOBAL  max |gpu - oracle| over the surface / max |oracle|      <= 1e-4
+  SURF_CELL    per cell, for cells >= 1e-3 of the peak: |d| / |oracle| <= 1e-4
+  NOISE_RTOL   noise power                                              1e-4
+  PTN_RTOL     peak-to-noise ratio                                       2e-4
+  Peak (row, col) must be identical unless the oracle's best and
+  second-best cells are within TIE_RTOL = 1e-5 of each other (a near-tie
+  that FP32 rounding can legitimately flip; the reference's own tests accept
+  either member of an exact mirror tie, test/secondary_code_search.jl:124-130).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SURF_GLOBAL = 1e-4
+SURF_CELL = 1e-4
+NOISE_RTOL = 1e-4
+PTN_RTOL = 2e-4
+TIE_RTOL = 1e-5
+
+
+def surface_errors(g: np.ndarray, o: np.ndarray):
+    o = np.asarray(o, dtype=np.float64)
+    g = np.asarray(g, dtype=np.float64)
+    mx = float(np.max(np.abs(o)))
+    d = np.abs(g - o)
+    glob = float(np.max(d) / mx) if mx > 0 else 0.0
+    big = np.abs(o) >= 1e-3 * mx
+    cell = float(np.max(d[big] / np.abs(o[big]))) if np.any(big) else 0.0
+    return glob, cell
+
+
+def is_near_tie(o: np.ndarray, row: int, col: int) -> bool:
+    """True when another cell of the oracle surface is within TIE_RTOL of the peak."""
+    o = np.asarray(o, dtype=np.float64)
+    pk = o[row, col]
+    flat = o.ravel(order="F").copy()
+    flat[col * o.shape[0] + row] = -np.inf
+    return bool(np.max(flat) >= pk * (1 - TIE_RTOL))
+
+
+def check_prn(res, ores, osurf, gsurf=None, *, interp=False, label=""):
+    """Compare one PRN's engine result against the oracle's."""
+    msgs = []
+    if gsurf is not None and osurf is not None:
+        glob, cell = surface_errors(gsurf, osurf)
+        assert glob <= SURF_GLOBAL, f"{label}: surface global rel err {glob:.3e}"
+        assert cell <= SURF_CELL, f"{label}: surface cell rel err {cell:.3e}"
+        msgs.append(f"surf {glob:.1e}/{cell:.1e}")
+    row, col = ores["row"], ores["col"]
+    tie = osurf is not None and is_near_tie(osurf, row, col)
+    es = getattr(res, "engine_stats", None)
+    if es is not None and not tie:
+        assert (es["row"], es["col"]) == (row, col), f"{label}: peak {(es['row'], es['col'])} vs oracle {(row, col)}"
+    noise_rel = abs(float(res.noise_power) - ores["noise"]) / abs(ores["noise"])
+    assert noise_rel <= NOISE_RTOL, f"{label}: noise rel err {noise_rel:.3e}"
+    ptn_rel = abs(float(res.peak_to_noise_ratio) - ores["ptn"]) / abs(ores["ptn"])
+    assert ptn_rel <= PTN_RTOL, f"{label}: ptn rel err {ptn_rel:.3e}"
+    if not tie:
+        if interp:
+            # interpolated estimates within 1e-3 bin of the oracle's
+            code_bin = res.system.get_code_frequency() / res.sampling_frequency
+            dc = abs(res.code_phase - ores["code_phase"])
+            dc = min(dc, res.system.get_code_length() - dc)
+            assert dc <= 1e-3 * code_bin + 1e-9, f"{label}: code phase {res.code_phase} vs {ores['code_phase']}"
+            step = float(res.dopplers[1] - res.dopplers[0])
+            assert abs(res.carrier_doppler - ores["doppler_hz"]) <= 1e-3 * step + 1e-9, \
+                f"{label}: doppler {res.carrier_doppler} vs {ores['doppler_hz']}"
+        else:
+            assert res.code_phase == ores["code_phase"], f"{label}: code phase {res.code_phase} vs {ores['code_phase']}"
+            assert res.carrier_doppler == ores["doppler_hz"], \
+                f"{label}: doppler {res.carrier_doppler} vs {ores['doppler_hz']}"
+    msgs.append(f"noise {noise_rel:.1e} ptn {ptn_rel:.1e}{' tie' if tie else ''}")
+    return "; ".join(msgs)
+
+
+def detection_consistent(res, ores, thr) -> bool:
+    """Identical CFAR decision unless the oracle's ratio is within PTN_RTOL of the threshold."""
+    g = float(res.peak_to_noise_ratio) > thr
+    o = ores["ptn"] > thr
+    if g == o:
+        return True
+    return abs(ores["ptn"] - thr) / thr <= PTN_RTOL

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Each case runs the same plan and the same seeded signal through both, at sizes
+the oracle finishes in seconds, and compares surfaces, peaks, noise, peak-to-
+noise ratios, labels and detections (tolerances in tests/parity.py).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from acquisition import (GPSL1CA, GPSL5I, GlobalMeanNoiseEstimator, acquire_, cfar_threshold, get_num_cells,
+                         is_detected, plan_acquire)
+from acquisition.signals import generate_test_signal, l1ca_5mhz_scene, l1ca_if_int16_scene
+from parity import check_prn, detection_consistent
+
+pytestmark = pytest.mark.gpu
+
+
+def run_case(system, fs, prns, signal, *, if_hz=0.0, interp=False, label="", **plan_kw):
+    plan = plan_acquire(system, fs, prns, **plan_kw)
+    try:
+        res = acquire_(plan, signal, prns, interm_freq=if_hz, subsample_interpolation=interp, store_power_bins=True)
+        ores, osurf = oracle.acquire(plan, signal, list(range(len(prns))), if_hz=if_hz,
+                                     subsample_interpolation=interp, surfaces=True)
+        out = []
+        for j, p in enumerate(prns):
+            msg = check_prn(res[j], ores[j], osurf[j], res[j].power_bins, interp=interp, label=f"{label} prn {p}")
+            thr = cfar_threshold(0.01, get_num_cells(res[j]),
+                                 num_noncoherent_integrations=res[j].num_noncoherent_integrations)
+            assert detection_consistent(res[j], ores[j], thr), f"{label} prn {p}: detection differs"
+            out.append(msg)
+        print(label, out[:3])
+        return plan, res, ores
+    finally:
+        plan.close()
+
+
+def l5i_clean(fs=10.24e6, prn=1, nper=10, r0=0, cp=0.0):
+    sysm = GPSL5I()
+    spc = math.ceil(sysm.code_length / sysm.code_frequency * fs)
+    primary = sysm.gen_code(spc, prn, fs, sysm.code_frequency, cp).astype(np.float32)
+    return np.concatenate([np.float32(sysm.secondary_value(prn, k + r0)) * primary for k in range(nper)]).astype(
+        np.complex64)
+
+
+def test_l5i_optout_kat_gpu():
+    """Reference KAT (test/secondary_code_search.jl:98-130) on the GPU."""
+    sig = l5i_clean()
+    plan, res, ores = run_case(GPSL5I(), 10.24e6, [1], sig, label="l5i-kat", min_doppler_coverage=1000.0,
+                               num_coherently_integrated_code_periods=10, use_secondary_code=False)
+    nim = res[0].power_bins
+    assert nim.shape == (40, 10240)
+    assert abs(float(nim.max()) / 1.146687232e9 - 1) < 1e-4
+    assert abs(float(nim.astype(np.float64).sum()) / 5.2553412608e10 - 1) < 1e-4
+    assert abs(float(nim[0, 99]) / 8.2944015625e4 - 1) < 1e-4
+    assert abs(float(nim[-1, -1]) / 2.318815625e4 - 1) < 1e-4
+    am = np.unravel_index(np.argmax(nim.ravel(order="F")), nim.shape, order="F")
+    assert (am[0] + 1, am[1] + 1) in ((17, 1), (25, 1))
+    assert abs(float(nim[16, 0]) / float(nim[24, 0]) - 1) < 1e-3
+
+
+def test_l1ca_5mhz_32prn_streamed():
+    """One-PRN 5 MHz scene searched over PRNs 1..32 (N_coh 1, N_nc 1)."""
+    sc = l1ca_5mhz_scene()
+    plan, res, ores = run_case(sc.system, sc.sampling_freq, sc.prns, sc.signal, interp=True, label="c1")
+    prn = sc.truth["prn"]
+    r = res[prn - 1]
+    assert is_detected(r)
+    d = abs(r.code_phase - sc.truth["code_phase"])
+    assert min(d, 1023 - d) < 1.0
+
+
+def test_multistep_simple_with_drift():
+    """N_coh 20, N_nc 8 at 2.048 MHz: the code-drift shift is non-zero at later steps."""
+    sysm = GPSL1CA()
+    fs = 2.048e6
+    g = generate_test_signal(sysm, 7, num_samples=8 * 20 * 2048, sampling_freq=fs, interm_freq=0.0, CN0=38,
+                             doppler=4321.0, code_phase=100.0, seed=11)
+    plan, res, ores = run_case(sysm, fs, [7, 8], g["signal"].astype(np.complex64), interp=True, label="drift",
+                               min_doppler_coverage=12000.0, num_coherently_integrated_code_periods=20,
+                               num_noncoherent_accumulations=8)
+    assert np.any(plan.code_drift_shifts != 0)
+    assert is_detected(res[0])
+
+
+def test_sign_path_two_data_bits_edge_search():
+    """N_coh 40 on L1 C/A (N_db 2), bit-edge search E 4, N_nc 2."""
+    sysm = GPSL1CA()
+    fs = 2.048e6
+    g = generate_test_signal(sysm, 3, num_samples=2 * 40 * 2048, sampling_freq=fs, interm_freq=0.0, CN0=36,
+                             doppler=-1500.0, code_phase=333.3, seed=12)
+    sig = g["signal"].copy()
+    ms = np.arange(sig.shape[0]) // 2048
+    sig *= np.where(((ms + 7) // 20) % 2 == 0, 1.0, -1.0)  # data bit flips every 20 ms, offset 7 ms
+    plan, res, ores = run_case(sysm, fs, [3, 4], sig.astype(np.complex64), label="sign", min_doppler_coverage=10000.0,
+                               num_coherently_integrated_code_periods=40, bit_edge_search_steps=4,
+                               num_noncoherent_accumulations=2)
+    assert plan.num_data_bits == 2 and plan.path == 1
+    assert is_detected(res[0])
+
+
+def test_sign_path_odd_doppler_bins():
+    """16.368 MHz, N_coh 5, E 5: ndop 55 (odd), the sign-path fold quirk."""
+    sysm = GPSL1CA()
+    fs = 16.368e6
+    g = generate_test_signal(sysm, 5, num_samples=5 * 16368, sampling_freq=fs, interm_freq=0.0, CN0=45,
+                             doppler=800.0, code_phase=512.25, seed=13)
+    plan, res, ores = run_case(sysm, fs, [5], g["signal"].astype(np.complex64), label="odd",
+                               min_doppler_coverage=5000.0, num_coherently_integrated_code_periods=5,
+                               bit_edge_search_steps=5)
+    assert plan.num_doppler_bins % 2 == 1 and plan.path == 1
+    # the sign-path fold never writes the last sorted row
+    assert float(np.max(res[0].power_bins[-1, :])) == 0.0
+
+
+def test_rotation_streamed_l5i():
+    """L5I NH10 rotation search at 12 MHz, N_coh 10, N_nc 1, r0 = 3."""
+    sysm = GPSL5I()
+    fs = 12e6
+    sig = l5i_clean(fs=fs, r0=3, cp=2000.0)
+    n = np.arange(sig.shape[0])
+    sig = (sig * np.exp(2j * np.pi * 1300.0 * n / fs)).astype(np.complex64)
+    noise = generate_test_signal(sysm, 1, num_samples=sig.shape[0], sampling_freq=fs, CN0=-200, seed=14)["signal"]
+    sig = (sig + 0.5 * noise / np.sqrt(fs)).astype(np.complex64)
+    plan, res, ores = run_case(sysm, fs, [1, 2], sig, interp=True, label="rot",
+                               num_coherently_integrated_code_periods=10)
+    assert plan.path == 2 and plan.num_secondary_rotations == 10
+    assert res[0].secondary_code_phase == 3
+
+
+def test_rotation_with_data_bits():
+    """L5I N 20 = 2 symbols: data-bit combos x rotations (Q 20)."""
+    sysm = GPSL5I()
+    fs = 12e6
+    spc = 12000
+    primary = sysm.gen_code(spc, 1, fs, sysm.code_frequency, 1234.5).astype(np.float32)
+    sig = np.concatenate([np.float32(sysm.secondary_value(1, k) * (1 if k < 10 else -1)) * primary
+                          for k in range(20)]).astype(np.complex64)
+    n = np.arange(sig.shape[0])
+    sig = (sig * np.exp(2j * np.pi * 1000.0 * n / fs)).astype(np.complex64)
+    plan, res, ores = run_case(sysm, fs, [1], sig, label="rot-data", num_coherently_integrated_code_periods=20)
+    assert plan.num_data_bits == 2 and plan.num_patterns == 20
+    assert res[0].secondary_code_phase == 0
+
+
+def test_rotation_multistep_drift():
+    """L5I rotation search, N_nc 4 at 5 kHz Doppler: drift on the rotation path."""
+    sysm = GPSL5I()
+    fs = 12e6
+    g = generate_test_signal(sysm, 1, num_samples=4 * 10 * 12000, sampling_freq=fs, interm_freq=0.0, CN0=45,
+                             doppler=5000.0, code_phase=1234.5, seed=15)
+    plan, res, ores = run_case(sysm, fs, [1], g["signal"].astype(np.complex64), interp=True, label="rot-drift",
+                               num_coherently_integrated_code_periods=10, num_noncoherent_accumulations=4)
+    assert np.any(plan.code_drift_shifts != 0)
+    assert is_detected(res[0])
+
+
+@pytest.mark.parametrize("nnc", [1, 2])
+def test_global_mean_estimator(nnc):
+    sysm = GPSL1CA()
+    fs = 4e6
+    g = generate_test_signal(sysm, 9, num_samples=nnc * 4000, sampling_freq=fs, interm_freq=0.0, CN0=44,
+                             doppler=2000.0, code_phase=700.0, seed=16)
+    run_case(sysm, fs, [9, 10, 11], g["signal"].astype(np.complex64), interp=True, label=f"gm{nnc}",
+             num_noncoherent_accumulations=nnc, noise_estimator=GlobalMeanNoiseEstimator())
+
+
+def test_int16_if_input():
+    """Real int16 IF samples (16.368 MHz, IF 4.092 MHz), N_coh 10, N_nc 2."""
+    sc = l1ca_if_int16_scene(n_ms=20, npresent=3, cn0_range=(42.0, 44.0))
+    prns = sc.present[:2] + [p for p in range(1, 33) if p not in sc.present][:1]
+    run_case(sc.system, sc.sampling_freq, prns, sc.signal, if_hz=sc.interm_freq, interp=True, label="int16-if",
+             num_coherently_integrated_code_periods=10, num_noncoherent_accumulations=2)
+
+
+def test_device_resident_signal_matches_host():
+    from acquisition.device import DeviceArray
+
+    sc = l1ca_5mhz_scene(seed=77)
+    plan = plan_acquire(sc.system, sc.sampling_freq, sc.prns)
+    try:
+        a = acquire_(plan, sc.signal, sc.prns)
+        a = [(r.engine_stats, float(r.peak_to_noise_ratio)) for r in a]
+        dev = DeviceArray(sc.signal)
+        b = acquire_(plan, dev, sc.prns)
+        b = [(r.engine_stats, float(r.peak_to_noise_ratio)) for r in b]
+        dev.free()
+        assert a == b
+    finally:
+        plan.close()
+
+
+def test_torch_tensor_signal_in_fresh_process():
+    """A CUDA torch tensor is read in place when torch is imported first (one
+    HIP runtime in the process)."""
+    import subprocess
+    import sys
+    import os
+
+    code = """
+import sys, torch
+sys.path[:0] = [{pkg!r}]
+import numpy as np
+from acquisition import acquire_, plan_acquire
+from acquisition.signals import l1ca_5mhz_scene
+sc = l1ca_5mhz_scene(seed=78)
+plan = plan_acquire(sc.system, sc.sampling_freq, sc.prns)
+a = [r.engine_stats for r in acquire_(plan, sc.signal, sc.prns)]
+b = [r.engine_stats for r in acquire_(plan, torch.from_numpy(sc.signal).to("cuda:0"), sc.prns)]
+plan.close()
+assert a == b
+print("torch-first ok")
+""".format(pkg=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "acquisition.jl_amd"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=170)
+    assert r.returncode == 0 and "torch-first ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_plan_reuse_and_errors():
+    from acquisition import ArgumentError
+
+    sysm = GPSL1CA()
+    fs = 2.048e6
+    plan = plan_acquire(sysm, fs, [1, 2])
+    try:
+        g1 = generate_test_signal(sysm, 1, num_samples=2048, sampling_freq=fs, interm_freq=0.0, CN0=50,
+                                  doppler=1000.0, code_phase=200.0, seed=1)
+        g2 = generate_test_signal(sysm, 2, num_samples=2048, sampling_freq=fs, interm_freq=0.0, CN0=50,
+                                  doppler=-2000.0, code_phase=600.0, seed=2)
+        r1 = acquire_(plan, g1["signal"], 1)
+        r2 = acquire_(plan, g2["signal"], 2)
+        assert is_detected(r1) and is_detected(r2)
+        assert abs(r1.code_phase - 200.0) < 1.0 and abs(r2.code_phase - 600.0) < 1.0
+        with pytest.raises(ArgumentError):
+            acquire_(plan, g1["signal"], [3])
+        with pytest.raises(ArgumentError):
+            acquire_(plan, g1["signal"][:1000], [1])
+    finally:
+        plan.close()
