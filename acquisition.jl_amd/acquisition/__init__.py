@@ -7,13 +7,14 @@ in-place call, as `!` does in Julia). Frequencies are plain Hz floats.
 from .estimators import AbstractNoiseEstimator, GlobalMeanNoiseEstimator, OppositeRowNoiseEstimator
 from .plan import AcquisitionPlan, ArgumentError, BATCH_FFT_THRESHOLD, plan_acquire
 from .cfar import cfar_threshold
-from .acquire import AcquisitionResults, acquire, acquire_, get_num_cells, is_detected
+from .acquire import (AcquisitionResults, Packed2BitSignal, acquire, acquire_, acquire_batch, get_num_cells,
+                      is_detected)
 from .systems import GNSSSystem, GPSL1CA, GPSL5I, E1Like, L5Like
 from ._native import EngineError
 
 __all__ = [
-    "acquire", "acquire_", "plan_acquire", "AcquisitionPlan", "AcquisitionResults",
+    "acquire", "acquire_", "acquire_batch", "plan_acquire", "AcquisitionPlan", "AcquisitionResults",
     "AbstractNoiseEstimator", "OppositeRowNoiseEstimator", "GlobalMeanNoiseEstimator",
-    "cfar_threshold", "get_num_cells", "is_detected", "ArgumentError", "EngineError",
+    "cfar_threshold", "get_num_cells", "is_detected", "Packed2BitSignal", "ArgumentError", "EngineError",
     "GNSSSystem", "GPSL1CA", "GPSL5I", "E1Like", "L5Like", "BATCH_FFT_THRESHOLD",
 ]

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libfmdbzp.so")
 ABI_VERSION = 1
 FMDBZP_OK = 0
 
-ELTYPE = {"cf32": 0, "cf64": 1, "i16": 2, "ci8": 3, "ci16": 4, "f32": 5, "f64": 6}
+ELTYPE = {"cf32": 0, "cf64": 1, "i16": 2, "ci8": 3, "ci16": 4, "f32": 5, "f64": 6, "c2bit": 7}
 FLAG_STORE_SURFACES = 0x1
 FLAG_DEVICE_SIGNAL = 0x2
 FLAG_DEVICE_OUTPUT = 0x4
@@ -63,9 +63,26 @@ class PrnStats(ctypes.Structure):
     ]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("plan", ctypes.c_void_p),
+        ("signal", ctypes.c_void_p),
+        ("eltype", ctypes.c_int32),
+        ("nprn", ctypes.c_int32),
+        ("n_samples", ctypes.c_int64),
+        ("interm_freq_hz", ctypes.c_double),
+        ("prn_index", ctypes.c_void_p),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("out_stats", ctypes.c_void_p),
+        ("out_surfaces", ctypes.c_void_p),
+    ]
+
+
 EXPORTED_SYMBOLS = (
     "fmdbzp_plan_create",
     "fmdbzp_acquire",
+    "fmdbzp_acquire_batch",
     "fmdbzp_plan_destroy",
     "fmdbzp_last_error",
     "fmdbzp_query",
@@ -86,6 +103,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fmdbzp_acquire.argtypes = [P, P, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, P, ctypes.c_int32,
                                    ctypes.c_uint32, P, P]
     lib.fmdbzp_acquire.restype = ctypes.c_int
+    lib.fmdbzp_acquire_batch.argtypes = [P, ctypes.c_int32]
+    lib.fmdbzp_acquire_batch.restype = ctypes.c_int
     lib.fmdbzp_plan_destroy.argtypes = [P]
     lib.fmdbzp_plan_destroy.restype = ctypes.c_int
     lib.fmdbzp_last_error.argtypes = []
@@ -107,6 +126,12 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         _lib = load_library()
     return _lib
+
+
+def acquire_batch(reqs):
+    """Submit a list of `Request`s as one batch."""
+    arr = (Request * len(reqs))(*reqs)
+    _check(lib().fmdbzp_acquire_batch(ctypes.cast(arr, ctypes.c_void_p), len(reqs)), "fmdbzp_acquire_batch")
 
 
 class EngineError(RuntimeError):
@@ -171,6 +196,19 @@ class NativePlan:
                                   ctypes.cast(surf_arg, ctypes.c_void_p) if surf_arg is not None else None)
         _check(rc, "fmdbzp_acquire")
         return stats
+
+    def request(self, signal_ptr: int, eltype: int, n_samples: int, if_hz: float, prn_index, flags: int,
+                surfaces=None):
+        """A filled `Request` plus the buffers it points to (kept alive by the caller)."""
+        idx = np.ascontiguousarray(prn_index, dtype=np.int32)
+        stats = (PrnStats * len(idx))()
+        surf_arg = None
+        if surfaces is not None:
+            surf_arg = (ctypes.c_void_p * len(idx))(*[ctypes.c_void_p(int(x)) if x else None for x in surfaces])
+        q = Request(self.handle, signal_ptr, eltype, len(idx), n_samples, float(if_hz), idx.ctypes.data, flags, 0,
+                    ctypes.cast(stats, ctypes.c_void_p).value,
+                    ctypes.cast(surf_arg, ctypes.c_void_p).value if surf_arg is not None else None)
+        return q, (idx, stats, surf_arg)
 
     def set_timing(self, enable: bool):
         _check(lib().fmdbzp_plan_set_timing(self.handle, 1 if enable else 0), "fmdbzp_plan_set_timing")

@@ -113,6 +113,48 @@ def _numpy_eltype(a: np.ndarray):
     raise ArgumentError(f"unsupported signal array dtype {a.dtype} with shape {a.shape}")
 
 
+class Packed2BitSignal:
+    """Packed 2-bit complex SDR samples (FMDBZP_C2BIT in include/fmdbzp.h): byte
+    i holds samples 2i (low nibble) and 2i+1 (high nibble); a nibble is I in
+    bits 0-1 and Q in bits 2-3, code c -> (c & 1 ? 3 : 1) * (c & 2 ? -1 : 1).
+    `packed` is a host uint8 array or a `device.DeviceArray` of those bytes."""
+
+    def __init__(self, packed, n: int):
+        self.packed = packed
+        self.n = int(n)
+
+    @staticmethod
+    def pack(values: np.ndarray) -> "Packed2BitSignal":
+        """Pack an (n, 2) array of I/Q values in {+1, +3, -1, -3}."""
+        v = np.asarray(values)
+        code = ((np.abs(v) == 3).astype(np.uint8) | ((v < 0).astype(np.uint8) << 1))
+        if not np.all(np.isin(np.abs(v), (1, 3))):
+            raise ArgumentError("2-bit samples must be +1, +3, -1 or -3")
+        nib = code[:, 0] | (code[:, 1] << 2)
+        n = nib.shape[0]
+        if n % 2:
+            nib = np.concatenate([nib, np.zeros(1, dtype=np.uint8)])
+        packed = (nib[0::2] | (nib[1::2] << 4)).astype(np.uint8)
+        return Packed2BitSignal(packed, n)
+
+    def unpack(self, count: Optional[int] = None) -> np.ndarray:
+        """Complex64 values of the first `count` samples."""
+        from .device import DeviceArray
+
+        n = self.n if count is None else min(count, self.n)
+        raw = self.packed.to_host((n + 1) // 2) if isinstance(self.packed, DeviceArray) else \
+            np.asarray(self.packed[:(n + 1) // 2])
+        nib = np.empty(2 * raw.shape[0], dtype=np.uint8)
+        nib[0::2] = raw & 15
+        nib[1::2] = raw >> 4
+        nib = nib[:n]
+
+        def val(c):
+            return np.where(c & 1, 3.0, 1.0) * np.where(c & 2, -1.0, 1.0)
+
+        return (val(nib & 3) + 1j * val((nib >> 2) & 3)).astype(np.complex64)
+
+
 class _SignalView:
     """Pointer, sample format and length of a host or device signal."""
 
@@ -120,7 +162,14 @@ class _SignalView:
         from .device import DeviceArray
 
         self.obj = signal
-        if isinstance(signal, DeviceArray):
+        if isinstance(signal, Packed2BitSignal):
+            self.kind, self.n = "c2bit", signal.n
+            self.device = isinstance(signal.packed, DeviceArray)
+            self.ptr = signal.packed.ptr if self.device else np.ascontiguousarray(signal.packed).ctypes.data
+            if not self.device:
+                self.obj = Packed2BitSignal(np.ascontiguousarray(signal.packed), signal.n)
+                self.ptr = self.obj.packed.ctypes.data
+        elif isinstance(signal, DeviceArray):
             self.kind, self.n = _numpy_eltype(np.empty((0,) + tuple(signal.shape[1:]), dtype=signal.dtype))
             self.n = signal.shape[0]
             self.device = True
@@ -159,6 +208,8 @@ class _SignalView:
         """ComplexF32 of the first n samples on the host."""
         from .device import DeviceArray
 
+        if isinstance(self.obj, Packed2BitSignal):
+            return self.obj.unpack(n)
         if isinstance(self.obj, DeviceArray):
             x = self.obj.to_host(n)
         else:
@@ -287,6 +338,18 @@ def acquire_(plan: AcquisitionPlan, signal, prns: Union[int, Sequence[int]], *, 
     if isinstance(prns, (int, np.integer)):
         return acquire_(plan, signal, [int(prns)], interm_freq=interm_freq,
                         subsample_interpolation=subsample_interpolation, store_power_bins=store_power_bins)[0]
+    prns, sv, idx, flags, bufs, surfaces = _prepare(plan, signal, prns, interm_freq, store_power_bins)
+    if not prns:
+        plan.acq_results_buf = []
+        return plan.acq_results_buf
+    stats = plan.native.acquire(sv.ptr, _native.ELTYPE[sv.kind], sv.n, float(interm_freq), idx, flags, surfaces)
+    results = _results_from_stats(plan, stats, prns, sv, interm_freq, subsample_interpolation, bufs)
+    plan.acq_results_buf = results
+    return results
+
+
+def _prepare(plan: AcquisitionPlan, signal, prns, interm_freq, store_power_bins):
+    """Validation shared by `acquire_` and `acquire_batch` (src/acquire.jl:62-72)."""
     prns = [int(p) for p in prns]
     if not all(p in plan.avail_prns for p in prns):
         raise ArgumentError(f"All requested PRNs must be in plan.avail_prns. Got: {prns}, "
@@ -301,18 +364,17 @@ def acquire_(plan: AcquisitionPlan, signal, prns: Union[int, Sequence[int]], *, 
             f"Signal has {sv.n} samples → {num_segments} full segments of {segment_length}, "
             f"but plan.num_noncoherent_accumulations={plan.num_noncoherent_accumulations}. "
             "Provide a longer signal.")
-    if not prns:
-        plan.acq_results_buf = []
-        return plan.acq_results_buf
     idx = [plan.avail_prns.index(p) for p in prns]
     flags = _native.FLAG_DEVICE_SIGNAL if sv.device else 0
-    surfaces = None
-    bufs = None
-    if store_power_bins:
+    bufs = surfaces = None
+    if store_power_bins and prns:
         flags |= _native.FLAG_STORE_SURFACES
         bufs = [_get_result_buffer(plan, i) for i in idx]
         surfaces = [b.ctypes.data for b in bufs]
-    stats = plan.native.acquire(sv.ptr, _native.ELTYPE[sv.kind], sv.n, float(interm_freq), idx, flags, surfaces)
+    return prns, sv, idx, flags, bufs, surfaces
+
+
+def _results_from_stats(plan, stats, prns, sv, interm_freq, subsample_interpolation, bufs):
     results = []
     for j, p in enumerate(prns):
         r = _extract(plan, stats[j], p, sv, float(interm_freq), subsample_interpolation,
@@ -323,8 +385,35 @@ def acquire_(plan: AcquisitionPlan, signal, prns: Union[int, Sequence[int]], *, 
                                                                 float(st.dop_left), float(st.dop_right)],
                               noise_sum=float(st.noise_sum), noise_cells=int(st.noise_cells))
         results.append(r)
-    plan.acq_results_buf = results
     return results
+
+
+def acquire_batch(requests):
+    """Run several acquisitions, possibly on different plans, as one submission
+    (`fmdbzp_acquire_batch`): requests on different plans overlap on the GPU.
+    Each request is (plan, signal, prns) or (plan, signal, prns, kwargs) with
+    the keywords of `acquire_`; returns one result list per request."""
+    prepared, natives, keep = [], [], []
+    for req in requests:
+        plan, signal, prns = req[0], req[1], req[2]
+        kw = dict(req[3]) if len(req) > 3 else {}
+        interm_freq = float(kw.get("interm_freq", 0.0))
+        interp = bool(kw.get("subsample_interpolation", False))
+        store = bool(kw.get("store_power_bins", False))
+        if isinstance(prns, (int, np.integer)):
+            prns = [int(prns)]
+        prns, sv, idx, flags, bufs, surfaces = _prepare(plan, signal, prns, interm_freq, store)
+        q, held = plan.native.request(sv.ptr, _native.ELTYPE[sv.kind], sv.n, interm_freq, idx, flags, surfaces)
+        natives.append(q)
+        keep.append((held, sv))
+        prepared.append((plan, prns, sv, interm_freq, interp, bufs, held[1]))
+    _native.acquire_batch(natives)
+    out = []
+    for plan, prns, sv, interm_freq, interp, bufs, stats in prepared:
+        res = _results_from_stats(plan, stats, prns, sv, interm_freq, interp, bufs)
+        plan.acq_results_buf = res
+        out.append(res)
+    return out
 
 
 def acquire(system, signal, sampling_freq: float, prns, *, interm_freq: float = 0.0,

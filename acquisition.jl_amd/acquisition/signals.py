@@ -159,3 +159,49 @@ def e1like_scene(seed: int = 0x5EED0004, n_periods: int = 100, nnc: int = 4, fs:
         x += amp * code * np.exp(1j * (2 * math.pi * fd * t / fs + ph))
         truth[prn] = dict(cn0=cn0, doppler=fd, code_phase=cp, secondary_phase=sec0)
     return Scene(x.astype(np.complex64), fs, 0.0, sysm, list(range(1, nprn + 1)), present, truth)
+
+
+def l1ca_quantized_scene(kind: str = "ci8", seed: int = 0x5EED0003, n_ms: int = 80, fs: float = 16.368e6,
+                         if_hz: float = 4.092e6, npresent: int = 8, sigma: float = 32.0,
+                         cn0_range=(30.0, 40.0)) -> Scene:
+    """The GPS L1 C/A scene as complex samples at IF `if_hz`, quantised to
+    Complex{Int8} with noise sigma `sigma` LSB per component (kind "ci8",
+    returned as an (n, 2) int8 array), or to 2 bits per component with levels
+    {+-1, +-3} and threshold sigma (kind "2bit", returned packed as
+    `Packed2BitSignal`). C/N0 = A^2 fs / (2 sigma^2) for a complex carrier."""
+    from .acquire import Packed2BitSignal
+
+    sysm = GPSL1CA()
+    n = int(round(fs * n_ms / 1000.0))
+    st = PhiloxStream(seed, 0)
+    perm = np.argsort(st.uniform(32))
+    present = sorted(int(p) + 1 for p in perm[:npresent])
+    t = np.arange(n, dtype=np.float64)
+    g = PhiloxStream(seed, 1).normal(2 * n)
+    x = (g[0::2] + 1j * g[1::2]) * sigma
+    truth = {}
+    for prn in present:
+        u = PhiloxStream(seed, 100 + prn).uniform(5)
+        cn0 = cn0_range[0] + (cn0_range[1] - cn0_range[0]) * u[0]
+        fd = -5000.0 + 10000.0 * u[1]
+        cp = 1023.0 * u[2]
+        ph = 2 * math.pi * u[3]
+        edge = int(u[4] * 20)
+        amp = math.sqrt(2 * sigma ** 2 * 10 ** (cn0 / 10) / fs)
+        code_freq = sysm.code_frequency + fd * sysm.get_code_center_frequency_ratio()
+        code = sysm.gen_code(n, prn, fs, code_freq, cp).astype(np.float64)
+        bits = PhiloxStream(seed, 200 + prn).signs(n_ms // 20 + 2).astype(np.float64)
+        ms = np.floor(t * 1000.0 / fs).astype(np.int64)
+        x += amp * bits[(ms + edge) // 20] * code * np.exp(1j * (2 * math.pi * (if_hz + fd) * t / fs + ph))
+        truth[prn] = dict(cn0=cn0, doppler=fd, code_phase=cp, edge_ms=edge)
+    if kind == "ci8":
+        q = np.stack([np.clip(np.rint(x.real), -128, 127), np.clip(np.rint(x.imag), -128, 127)], axis=1)
+        sig = q.astype(np.int8)
+    elif kind == "2bit":
+        def lvl(v):
+            return np.where(v >= 0, np.where(v >= sigma, 3, 1), np.where(v <= -sigma, -3, -1))
+
+        sig = Packed2BitSignal.pack(np.stack([lvl(x.real), lvl(x.imag)], axis=1).astype(np.int8))
+    else:
+        raise ValueError(kind)
+    return Scene(sig, fs, if_hz, sysm, list(range(1, 33)), present, truth)

@@ -148,25 +148,43 @@ __device__ __forceinline__ void bfly(float2* v)
     else bfly8<INV>(v);
 }
 
-// One Stockham stage with a compile-time radix over `nbatch` transforms of
-// length n laid out at stride `bstride` in LDS.
-template <int R, bool INV>
-__device__ __forceinline__ void stage_fixed(const float2* __restrict__ src, float2* __restrict__ dst,
-                                            int n, int Ns, int nbatch, int bstride,
-                                            const float2* __restrict__ tw, int tid, int nthr)
+// ---------------------------------------------------------------------------
+// Functor-driven transform: the first stage reads its butterfly inputs through
+// `gin(t, i)` (element i of transform t, e.g. straight from global memory or
+// computed on the fly) and the last stage hands its outputs to
+// `gout(t, i, v)` (element i of the transform's natural-order result), so the
+// LDS round trips before the first and after the last stage disappear.
+// Intermediate stages ping-pong between the LDS buffers a and b (stride bst).
+// COLFAST selects the thread mapping: transform index fastest (consecutive
+// lanes work on consecutive transforms at the same element, which coalesces
+// column-strided global data) or element fastest (one long transform).
+// ---------------------------------------------------------------------------
+struct LdsIn {
+    const float2* buf;
+    int bst;
+    __device__ __forceinline__ float2 operator()(int t, int i) const { return buf[t * bst + i]; }
+};
+struct LdsOut {
+    float2* buf;
+    int bst;
+    __device__ __forceinline__ void operator()(int t, int i, float2 v) const { buf[t * bst + i] = v; }
+};
+
+template <int R, bool INV, bool COLFAST, class In, class Out>
+__device__ __forceinline__ void stage_io(const In& in, const Out& out, int n, int Ns, int nbatch,
+                                         const float2* __restrict__ tw, int tid, int nthr)
 {
     const int m = n / R;
     const int tstride = n / (Ns * R);
     const int total = nbatch * m;
     for (int idx = tid; idx < total; idx += nthr) {
-        const int t = idx / m;
-        const int j = idx - t * m;
-        const float2* in = src + t * bstride;
-        float2* out = dst + t * bstride;
+        int t, j;
+        if (COLFAST) { j = idx / nbatch; t = idx - j * nbatch; }
+        else { t = idx / m; j = idx - t * m; }
         const int k = j % Ns;
         float2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = in[j + r * m];
+        for (int r = 0; r < R; r++) v[r] = in(t, j + r * m);
         if (Ns > 1) {
             const int ts = k * tstride;
 #pragma unroll
@@ -175,66 +193,78 @@ __device__ __forceinline__ void stage_fixed(const float2* __restrict__ src, floa
         bfly<R, INV>(v);
         const int base = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; r++) out[base + r * Ns] = v[r];
+        for (int r = 0; r < R; r++) out(t, base + r * Ns, v[r]);
     }
 }
 
-// Stage with an arbitrary prime radix: direct R-point DFT per output.
-template <bool INV>
-__device__ void stage_generic(const float2* __restrict__ src, float2* __restrict__ dst, int n, int R,
-                              int Ns, int nbatch, int bstride, const float2* __restrict__ tw,
-                              int tid, int nthr)
+template <bool INV, bool COLFAST, class In, class Out>
+__device__ void stage_io_generic(const In& in, const Out& out, int n, int R, int Ns, int nbatch,
+                                 const float2* __restrict__ tw, int tid, int nthr)
 {
     const int m = n / R;
     const int tstride = n / (Ns * R);
     const int total = nbatch * m * R;
     for (int idx = tid; idx < total; idx += nthr) {
-        const int t = idx / (m * R);
-        const int rem = idx - t * (m * R);
-        const int s = rem / m;       // output index within the butterfly
-        const int j = rem - s * m;   // butterfly
-        const float2* in = src + t * bstride;
-        float2* out = dst + t * bstride;
+        int t, rem;
+        if (COLFAST) { rem = idx / nbatch; t = idx - rem * nbatch; }
+        else { t = idx / (m * R); rem = idx - t * (m * R); }
+        const int s = rem / m;
+        const int j = rem - s * m;
         const int k = j % Ns;
         float2 acc = make_float2(0.f, 0.f);
         for (int r = 0; r < R; r++) {
-            float2 v = in[j + r * m];
+            float2 v = in(t, j + r * m);
             if (Ns > 1 && r > 0) v = cmulf(v, twid<INV>(tw, r * k * tstride));
-            // W_R^{r s} = tw[((r s) mod R) * (n / R)]
             const int e = (r * s) % R;
             if (e) v = cmulf(v, twid<INV>(tw, e * m));
             acc = caddf(acc, v);
         }
-        const int base = (j - k) * R + k;
-        out[base + s * Ns] = acc;
+        out(t, (j - k) * R + k + s * Ns, acc);
     }
 }
 
-// Run the whole transform. Data starts in `a` (all threads must have written
-// it and passed a barrier); returns the buffer (a or b) holding the result,
-// after a final barrier.
-template <bool INV>
-__device__ float2* fft_lds(float2* a, float2* b, const FftPlanDev& p, const float2* __restrict__ tw,
-                           int nbatch, int bstride, int tid, int nthr)
+template <bool INV, bool COLFAST, class In, class Out>
+__device__ __forceinline__ void stage_any(int R, const In& in, const Out& out, int n, int Ns, int nbatch,
+                                          const float2* __restrict__ tw, int tid, int nthr)
 {
+    switch (R) {
+    case 2: stage_io<2, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
+    case 3: stage_io<3, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
+    case 4: stage_io<4, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
+    case 5: stage_io<5, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
+    case 8: stage_io<8, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
+    default: stage_io_generic<INV, COLFAST>(in, out, n, R, Ns, nbatch, tw, tid, nthr); break;
+    }
+}
+
+// Whole transform through gin/gout. Every thread of the block must call it.
+// The caller must place a barrier before it if gin reads LDS written by other
+// threads, and after it before reading what gout wrote to LDS.
+template <bool INV, bool COLFAST, class GIn, class GOut>
+__device__ void fft_run(float2* a, float2* b, const FftPlanDev& p, const float2* __restrict__ tw, int nbatch,
+                        int bst, int tid, int nthr, const GIn& gin, const GOut& gout)
+{
+    const int n = p.n;
+    if (p.nst == 0) {
+        for (int t = tid; t < nbatch; t += nthr) gout(t, 0, gin(t, 0));
+        return;
+    }
+    if (p.nst == 1) {
+        stage_any<INV, COLFAST>(p.radix[0], gin, gout, n, 1, nbatch, tw, tid, nthr);
+        return;
+    }
+    stage_any<INV, COLFAST>(p.radix[0], gin, LdsOut{a, bst}, n, 1, nbatch, tw, tid, nthr);
+    __syncthreads();
+    int Ns = p.radix[0];
     float2* src = a;
     float2* dst = b;
-    int Ns = 1;
-    for (int s = 0; s < p.nst; s++) {
-        const int R = p.radix[s];
-        switch (R) {
-        case 2: stage_fixed<2, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
-        case 3: stage_fixed<3, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
-        case 4: stage_fixed<4, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
-        case 5: stage_fixed<5, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
-        case 8: stage_fixed<8, INV>(src, dst, p.n, Ns, nbatch, bstride, tw, tid, nthr); break;
-        default: stage_generic<INV>(src, dst, p.n, R, Ns, nbatch, bstride, tw, tid, nthr); break;
-        }
+    for (int s = 1; s < p.nst - 1; s++) {
+        stage_any<INV, COLFAST>(p.radix[s], LdsIn{src, bst}, LdsOut{dst, bst}, n, Ns, nbatch, tw, tid, nthr);
         __syncthreads();
         float2* t = src;
         src = dst;
         dst = t;
-        Ns *= R;
+        Ns *= p.radix[s];
     }
-    return src;
+    stage_any<INV, COLFAST>(p.radix[p.nst - 1], LdsIn{src, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
 }

@@ -17,6 +17,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -132,7 +133,14 @@ struct fmdbzp_plan {
     int32_t* d_req = nullptr;
     size_t req_cap = 0;
     fmdbzp_prn_stats* d_stats = nullptr;
-    size_t stats_cap = 0;
+    // pinned host staging, so copies stay asynchronous to the host
+    int32_t* h_req = nullptr;
+    fmdbzp_prn_stats* h_stats = nullptr;
+    void* h_signal = nullptr;
+    size_t h_signal_cap = 0;
+    // the request whose statistics are staged in h_stats, not yet returned
+    fmdbzp_prn_stats* pending_out = nullptr;
+    int32_t pending_n = 0;
     float* d_export = nullptr;
     int batch = 1;
     int LB = 1, bstride = 1;
@@ -159,7 +167,7 @@ size_t elsize(int eltype)
     case FMDBZP_CI16: return 4;
     case FMDBZP_F32: return 4;
     case FMDBZP_F64: return 8;
-    default: return 0;
+    default: return 0;  // FMDBZP_C2BIT is sized in signal_bytes()
     }
 }
 
@@ -237,6 +245,8 @@ int validate_desc(const fmdbzp_plan_desc* d)
     if (d->path == FMDBZP_PATH_ROTATION &&
         d->num_patterns != (1 << (d->num_data_bits - 1)) * d->num_secondary_rotations)
         return set_err(FMDBZP_EINVAL, "rotation path needs Q = 2^(N_db-1) * R patterns");
+    if (d->path == FMDBZP_PATH_ROTATION && d->num_coh > 256)
+        return set_err(FMDBZP_EINVAL, "rotation path supports num_coh <= 256");
     if (d->path == FMDBZP_PATH_SIGN && d->num_patterns != (1 << (d->num_data_bits - 1)))
         return set_err(FMDBZP_EINVAL, "sign path needs Q = 2^(N_db-1) patterns");
     if (d->path != FMDBZP_PATH_ROTATION && d->num_secondary_rotations != 1)
@@ -248,8 +258,8 @@ int validate_desc(const fmdbzp_plan_desc* d)
     // LDS: the F-point transforms need two F-entry buffers in one workgroup
     if ((size_t)d->fft_size * 2 * sizeof(float2) > kLdsMax)
         return set_err(FMDBZP_EINVAL, "fft_size %d exceeds the LDS transform limit", d->fft_size);
-    // the column stage needs three ndop-entry complex buffers + one float buffer
-    if ((size_t)(d->num_doppler_bins | 1) * 28 > kLdsMax)
+    // the column stage needs three ndop-entry complex buffers + two float rows
+    if ((size_t)(d->num_doppler_bins | 1) * 32 > kLdsMax)
         return set_err(FMDBZP_EINVAL, "num_doppler_bins %d exceeds the LDS column limit", d->num_doppler_bins);
     int64_t seglen = (int64_t)d->num_coh * d->samples_per_code;
     if (seglen * d->num_noncoherent > ((int64_t)1 << 40)) return set_err(FMDBZP_EINVAL, "signal too long");
@@ -298,6 +308,9 @@ void free_plan(fmdbzp_plan* P)
     hipFree(P->d_req);
     hipFree(P->d_stats);
     hipFree(P->d_export);
+    hipHostFree(P->h_req);
+    hipHostFree(P->h_stats);
+    hipHostFree(P->h_signal);
     for (auto& e : P->pending) { hipEventDestroy(e.start); hipEventDestroy(e.stop); }
     for (auto e : P->pool) hipEventDestroy(e);
     if (P->stream) hipStreamDestroy(P->stream);
@@ -387,11 +400,16 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
 
     // column-stage geometry: LB columns per workgroup, padded column stride
     P->bstride = ndop | 1;
+    // simple: two ping-pong columns; sign/rotation: input column, two
+    // ping-pong columns, stage and raw-power rows
     const size_t per_col = d.path == FMDBZP_PATH_SIMPLE ? (size_t)P->bstride * 2 * sizeof(float2)
                                                          : (size_t)P->bstride * 3 * sizeof(float2) +
-                                                               (size_t)ndop * sizeof(float);
-    int LB = (int)(kLdsDefault / per_col);
+                                                               (size_t)ndop * 2 * sizeof(float);
+    // the sign/rotation kernels add ~1.3 KB of static LDS to the dynamic part
+    const size_t budget = d.path == FMDBZP_PATH_SIMPLE ? kLdsDefault : kLdsDefault - 4096;
+    int LB = (int)(budget / per_col);
     if (LB > 64) LB = 64;
+    if (LB >= 16) LB &= ~7;
     if (LB < 1) LB = 1;
     if (LB > bs) LB = bs;
     P->LB = LB;
@@ -508,24 +526,19 @@ int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launc
     return FMDBZP_K_NUM_CLASSES;
 }
 
-int fmdbzp_acquire(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_t n_samples,
-                   double interm_freq_hz, const int32_t* prn_index, int32_t nreq, uint32_t flags,
-                   fmdbzp_prn_stats* out_stats, float* const* out_surfaces)
+}  // extern "C"
+
+namespace {
+
+// Enqueue one search on the plan's stream; statistics land in the plan's
+// pinned staging buffer and are returned by acquire_finish. The caller holds
+// the plan's mutex.
+int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t n_samples,
+                    double interm_freq_hz, const int32_t* prn_index, int32_t nreq, uint32_t flags,
+                    fmdbzp_prn_stats* out_stats, float* const* out_surfaces)
 {
-    g_err.clear();
-    if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
-    fmdbzp_plan* P = plan;
-    std::lock_guard<std::mutex> lk(P->mu);
     const fmdbzp_plan_desc& d = P->d;
-    if (!signal || !prn_index || !out_stats) return set_err(FMDBZP_EINVAL, "null argument to fmdbzp_acquire");
-    if (nreq <= 0) return set_err(FMDBZP_EINVAL, "nreq must be positive");
-    const size_t es = elsize(eltype);
-    if (!es) return set_err(FMDBZP_EINVAL, "unknown eltype %d", eltype);
     const bool store = (flags & FMDBZP_FLAG_STORE_SURFACES) != 0;
-    if (store && !out_surfaces) return set_err(FMDBZP_EINVAL, "STORE_SURFACES without surface pointers");
-    for (int i = 0; i < nreq; i++)
-        if (prn_index[i] < 0 || prn_index[i] >= P->nprn)
-            return set_err(FMDBZP_EINVAL, "prn_index[%d] = %d out of range [0, %d)", i, prn_index[i], P->nprn);
     const int F = d.fft_size, ndop = d.num_doppler_bins, nb = d.num_blocks, bs = d.block_size;
     const int spc = d.samples_per_code, spc_eff = d.samples_per_code_eff, nnc = d.num_noncoherent;
     const int64_t seglen = (int64_t)d.num_coh * spc;
@@ -535,7 +548,8 @@ int fmdbzp_acquire(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_
     HIP_TRY(hipSetDevice(P->device));
 
     // signal: only the N_nc segments the search reads cross the bus
-    const size_t sig_bytes = (size_t)seglen * nnc * es;
+    const size_t nsig = (size_t)seglen * nnc;
+    const size_t sig_bytes = eltype == FMDBZP_C2BIT ? (nsig + 1) / 2 : nsig * elsize(eltype);
     const void* d_sig = signal;
     if (!(flags & FMDBZP_FLAG_DEVICE_SIGNAL)) {
         if (P->signal_cap < sig_bytes) {
@@ -545,20 +559,35 @@ int fmdbzp_acquire(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_
             HIP_TRY(hipMalloc(&P->d_signal, sig_bytes));
             P->signal_cap = sig_bytes;
         }
-        HIP_TRY(hipMemcpyAsync(P->d_signal, signal, sig_bytes, hipMemcpyHostToDevice, P->stream));
+        if (P->h_signal_cap < sig_bytes) {
+            hipHostFree(P->h_signal);
+            P->h_signal = nullptr;
+            P->h_signal_cap = 0;
+            HIP_TRY(hipHostMalloc(&P->h_signal, sig_bytes, hipHostMallocDefault));
+            P->h_signal_cap = sig_bytes;
+        }
+        memcpy(P->h_signal, signal, sig_bytes);
+        HIP_TRY(hipMemcpyAsync(P->d_signal, P->h_signal, sig_bytes, hipMemcpyHostToDevice, P->stream));
         d_sig = P->d_signal;
     }
     if (P->req_cap < (size_t)nreq) {
         hipFree(P->d_req);
         hipFree(P->d_stats);
+        hipHostFree(P->h_req);
+        hipHostFree(P->h_stats);
         P->d_req = nullptr;
         P->d_stats = nullptr;
+        P->h_req = nullptr;
+        P->h_stats = nullptr;
         P->req_cap = 0;
         HIP_TRY(hipMalloc((void**)&P->d_req, sizeof(int32_t) * nreq));
         HIP_TRY(hipMalloc((void**)&P->d_stats, sizeof(fmdbzp_prn_stats) * nreq));
+        HIP_TRY(hipHostMalloc((void**)&P->h_req, sizeof(int32_t) * nreq, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&P->h_stats, sizeof(fmdbzp_prn_stats) * nreq, hipHostMallocDefault));
         P->req_cap = (size_t)nreq;
     }
-    HIP_TRY(hipMemcpyAsync(P->d_req, prn_index, sizeof(int32_t) * nreq, hipMemcpyHostToDevice, P->stream));
+    memcpy(P->h_req, prn_index, sizeof(int32_t) * nreq);
+    HIP_TRY(hipMemcpyAsync(P->d_req, P->h_req, sizeof(int32_t) * nreq, hipMemcpyHostToDevice, P->stream));
     if (store && !P->d_export && !(flags & FMDBZP_FLAG_DEVICE_OUTPUT))
         HIP_TRY(hipMalloc((void**)&P->d_export, sizeof(float) * (size_t)ndop * spc_eff));
 
@@ -651,11 +680,88 @@ int fmdbzp_acquire(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_
             }
         }
     }
-    HIP_TRY(hipMemcpyAsync(out_stats, P->d_stats, sizeof(fmdbzp_prn_stats) * nreq, hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(P->h_stats, P->d_stats, sizeof(fmdbzp_prn_stats) * nreq, hipMemcpyDeviceToHost,
                            P->stream));
+    P->pending_out = out_stats;
+    P->pending_n = nreq;
+    return FMDBZP_OK;
+}
+
+// Wait for the plan's stream and return the staged statistics.
+int acquire_finish(fmdbzp_plan* P)
+{
+    HIP_TRY(hipSetDevice(P->device));
     HIP_TRY(hipStreamSynchronize(P->stream));
+    if (P->pending_out) memcpy(P->pending_out, P->h_stats, sizeof(fmdbzp_prn_stats) * P->pending_n);
+    P->pending_out = nullptr;
+    P->pending_n = 0;
     if (P->timing) collect_timing(P);
     return FMDBZP_OK;
+}
+
+int validate_request(fmdbzp_plan* P, const void* signal, int32_t eltype, int32_t nreq, uint32_t flags,
+                     const int32_t* prn_index, fmdbzp_prn_stats* out_stats, float* const* out_surfaces)
+{
+    if (!signal || !prn_index || !out_stats) return set_err(FMDBZP_EINVAL, "null argument to fmdbzp_acquire");
+    if (nreq <= 0) return set_err(FMDBZP_EINVAL, "nreq must be positive");
+    if (!elsize(eltype) && eltype != FMDBZP_C2BIT) return set_err(FMDBZP_EINVAL, "unknown eltype %d", eltype);
+    if ((flags & FMDBZP_FLAG_STORE_SURFACES) && !out_surfaces)
+        return set_err(FMDBZP_EINVAL, "STORE_SURFACES without surface pointers");
+    for (int i = 0; i < nreq; i++)
+        if (prn_index[i] < 0 || prn_index[i] >= P->nprn)
+            return set_err(FMDBZP_EINVAL, "prn_index[%d] = %d out of range [0, %d)", i, prn_index[i], P->nprn);
+    return FMDBZP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmdbzp_acquire(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_t n_samples,
+                   double interm_freq_hz, const int32_t* prn_index, int32_t nreq, uint32_t flags,
+                   fmdbzp_prn_stats* out_stats, float* const* out_surfaces)
+{
+    g_err.clear();
+    if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
+    std::lock_guard<std::mutex> lk(plan->mu);
+    int rc = validate_request(plan, signal, eltype, nreq, flags, prn_index, out_stats, out_surfaces);
+    if (rc) return rc;
+    rc = acquire_enqueue(plan, signal, eltype, n_samples, interm_freq_hz, prn_index, nreq, flags, out_stats,
+                         out_surfaces);
+    int rc2 = acquire_finish(plan);
+    return rc ? rc : rc2;
+}
+
+int fmdbzp_acquire_batch(const fmdbzp_request* reqs, int32_t nreqs)
+{
+    g_err.clear();
+    if (!reqs || nreqs <= 0) return set_err(FMDBZP_EINVAL, "empty batch");
+    std::vector<fmdbzp_plan*> plans;
+    for (int i = 0; i < nreqs; i++) {
+        if (!reqs[i].plan) return set_err(FMDBZP_EHANDLE, "request %d has a null plan", i);
+        plans.push_back(reqs[i].plan);
+    }
+    std::sort(plans.begin(), plans.end());
+    plans.erase(std::unique(plans.begin(), plans.end()), plans.end());
+    // lock every plan of the batch in address order
+    std::vector<std::unique_lock<std::mutex>> locks;
+    for (auto* p : plans) locks.emplace_back(p->mu);
+    int rc = FMDBZP_OK;
+    for (int i = 0; i < nreqs && rc == FMDBZP_OK; i++) {
+        const fmdbzp_request& q = reqs[i];
+        rc = validate_request(q.plan, q.signal, q.eltype, q.nprn, q.flags, q.prn_index, q.out_stats, q.out_surfaces);
+        if (rc) break;
+        // a plan used twice returns its first request before the second reuses its workspace
+        if (q.plan->pending_out) rc = acquire_finish(q.plan);
+        if (rc) break;
+        rc = acquire_enqueue(q.plan, q.signal, q.eltype, q.n_samples, q.interm_freq_hz, q.prn_index, q.nprn,
+                             q.flags, q.out_stats, q.out_surfaces);
+    }
+    for (auto* p : plans) {
+        int r2 = acquire_finish(p);
+        if (rc == FMDBZP_OK) rc = r2;
+    }
+    return rc;
 }
 
 }  // extern "C"

@@ -36,6 +36,14 @@ __device__ __forceinline__ float2 load_sample(const void* __restrict__ sig, int 
     }
     case FMDBZP_F32: return make_float2(((const float*)sig)[i], 0.f);
     case FMDBZP_F64: return make_float2((float)((const double*)sig)[i], 0.f);
+    case FMDBZP_C2BIT: {
+        const unsigned b = ((const uint8_t*)sig)[i >> 1];
+        const unsigned nib = (i & 1) ? (b >> 4) : (b & 15u);
+        const unsigned ci = nib & 3u, cq = (nib >> 2) & 3u;
+        const float vi = (ci & 1u ? 3.f : 1.f) * (ci & 2u ? -1.f : 1.f);
+        const float vq = (cq & 1u ? 3.f : 1.f) * (cq & 2u ? -1.f : 1.f);
+        return make_float2(vi, vq);
+    }
     default: return make_float2(0.f, 0.f);
     }
 }
@@ -51,22 +59,29 @@ __device__ __forceinline__ float abs2_exact(float2 z)
 // Plan time: conjugated code sub-block FFTs (src/plan.jl:230-251).
 // grid (nb, nprn)
 // ---------------------------------------------------------------------------
+struct CodeIn {
+    const int8_t* code;
+    int bs;
+    __device__ __forceinline__ float2 operator()(int, int i) const
+    {
+        return make_float2(i < bs ? (float)code[i] : 0.f, 0.f);
+    }
+};
+struct ConjOut {
+    float2* out;
+    __device__ __forceinline__ void operator()(int, int i, float2 v) const { out[i] = make_float2(v.x, -v.y); }
+};
+
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_code_fft(const int8_t* __restrict__ codes, float2* __restrict__ prnconj, FftPlanDev pf,
            const float2* __restrict__ twF, int spc, int nb, int bs)
 {
     extern __shared__ float2 lds[];
     const int F = pf.n;
-    float2* a = lds;
-    float2* b = lds + F;
     const int blk = blockIdx.x, prn = blockIdx.y;
-    const int tid = threadIdx.x;
     const int8_t* code = codes + (size_t)prn * spc + (size_t)blk * bs;
-    for (int i = tid; i < F; i += FMDBZP_NTHR) a[i] = make_float2(i < bs ? (float)code[i] : 0.f, 0.f);
-    __syncthreads();
-    float2* res = fft_lds<false>(a, b, pf, twF, 1, F, tid, FMDBZP_NTHR);
     float2* out = prnconj + ((size_t)prn * nb + blk) * F;
-    for (int i = tid; i < F; i += FMDBZP_NTHR) out[i] = make_float2(res[i].x, -res[i].y);
+    fft_run<false, false>(lds, lds + F, pf, twF, 1, F, threadIdx.x, FMDBZP_NTHR, CodeIn{code, bs}, ConjOut{out});
 }
 
 // ---------------------------------------------------------------------------
@@ -77,6 +92,36 @@ k_code_fft(const int8_t* __restrict__ codes, float2* __restrict__ prnconj, FftPl
 // sin/cos of that Float32 argument are evaluated in FP64 and rounded.
 // grid (ncoh*nb, nnc)
 // ---------------------------------------------------------------------------
+struct DownconvIn {
+    const void* sig;
+    int eltype;
+    int64_t seg0, seglen, blk0;
+    int two_bs;
+    int do_if;
+    float off, step;
+    __device__ __forceinline__ float2 operator()(int, int i) const
+    {
+        float2 v = make_float2(0.f, 0.f);
+        if (i < two_bs) {
+            int64_t j = blk0 + i;
+            if (j >= seglen) j -= seglen;  // last double block wraps (coherent_integration.jl:44-50)
+            v = load_sample(sig, eltype, seg0 + j);
+            if (do_if) {
+#pragma clang fp contract(off)
+                const float ph = off + (float)j * step;
+                const double dph = (double)ph;
+                const float s = (float)sin(dph), c = (float)cos(dph);
+                v = make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+            }
+        }
+        return v;
+    }
+};
+struct PlainOut {
+    float2* out;
+    __device__ __forceinline__ void operator()(int, int i, float2 v) const { out[i] = v; }
+};
+
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_signal_fft(const void* __restrict__ sig, int eltype, float2* __restrict__ sigfft, FftPlanDev pf,
              const float2* __restrict__ twF, int bs, int64_t seglen, int ndop, int do_if,
@@ -84,41 +129,37 @@ k_signal_fft(const void* __restrict__ sig, int eltype, float2* __restrict__ sigf
 {
     extern __shared__ float2 lds[];
     const int F = pf.n;
-    float2* a = lds;
-    float2* b = lds + F;
     const int gk = blockIdx.x, step = blockIdx.y;
-    const int tid = threadIdx.x;
     const int64_t seg0 = (int64_t)step * seglen;
-    const float off = __fmul_rn((float)seg0, phase_step);  // Float32(seg_start-1) * step
-    for (int i = tid; i < F; i += FMDBZP_NTHR) {
-        float2 v = make_float2(0.f, 0.f);
-        if (i < 2 * bs) {
-            int64_t j = (int64_t)gk * bs + i;
-            if (j >= seglen) j -= seglen;  // last double block wraps (coherent_integration.jl:44-50)
-            v = load_sample(sig, eltype, seg0 + j);
-            if (do_if) {
-#pragma clang fp contract(off)
-                const float ph = off + (float)j * phase_step;
-                const double dph = (double)ph;
-                const float s = (float)sin(dph), c = (float)cos(dph);
-                v = make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
-            }
-        }
-        a[i] = v;
-    }
-    __syncthreads();
-    float2* res = fft_lds<false>(a, b, pf, twF, 1, F, tid, FMDBZP_NTHR);
+    DownconvIn in{sig, eltype, seg0, seglen, (int64_t)gk * bs, 2 * bs, do_if,
+                  __fmul_rn((float)seg0, phase_step) /* Float32(seg_start-1) * step */, phase_step};
     float2* out = sigfft + ((size_t)step * ndop + gk) * F;
-    for (int i = tid; i < F; i += FMDBZP_NTHR) out[i] = res[i];
+    fft_run<false, false>(lds, lds + F, pf, twF, 1, F, threadIdx.x, FMDBZP_NTHR, in, PlainOut{out});
 }
 
 // ---------------------------------------------------------------------------
 // Block correlation: `_build_coherent_tile!` (src/coherent_integration.jl:136-171).
 // Row k of column block r correlates signal double block k with code sub-block
-// (k mod nb + r) mod nb: pointwise product in the frequency domain, inverse
-// FFT (unnormalised), times Float32(1/F), first bs lags kept.
+// (k mod nb + r) mod nb: pointwise product in the frequency domain (formed in
+// registers by the first FFT stage), inverse FFT (unnormalised), times
+// Float32(1/F); the last stage stores only the first bs lags.
 // grid (ndop, nb, batch)
 // ---------------------------------------------------------------------------
+struct CorrIn {
+    const float2* S;
+    const float2* P;
+    __device__ __forceinline__ float2 operator()(int, int i) const { return cmulf(S[i], P[i]); }
+};
+struct LagOut {
+    float2* out;
+    int bs;
+    float inv_f;
+    __device__ __forceinline__ void operator()(int, int i, float2 v) const
+    {
+        if (i < bs) out[i] = make_float2(v.x * inv_f, v.y * inv_f);
+    }
+};
+
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
             const int32_t* __restrict__ req, float2* __restrict__ tile, FftPlanDev pf,
@@ -126,24 +167,18 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
 {
     extern __shared__ float2 lds[];
     const int F = pf.n;
-    float2* a = lds;
-    float2* b = lds + F;
     const int k = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
-    const int tid = threadIdx.x;
     const int prn = req[p];
     const int pb = (k % nb + r) % nb;
-    const float2* S = sigfft_step + (size_t)k * F;
-    const float2* P = prnconj + ((size_t)prn * nb + pb) * F;
-    for (int i = tid; i < F; i += FMDBZP_NTHR) a[i] = cmulf(S[i], P[i]);
-    __syncthreads();
-    float2* res = fft_lds<true>(a, b, pf, twF, 1, F, tid, FMDBZP_NTHR);
-    float2* out = tile + (((size_t)p * nb + r) * ndop + k) * bs;
-    for (int i = tid; i < bs; i += FMDBZP_NTHR) out[i] = make_float2(res[i].x * inv_f, res[i].y * inv_f);
+    CorrIn in{sigfft_step + (size_t)k * F, prnconj + ((size_t)prn * nb + pb) * F};
+    LagOut out{tile + (((size_t)p * nb + r) * ndop + k) * bs, bs, inv_f};
+    fft_run<true, false>(lds, lds + F, pf, twF, 1, F, threadIdx.x, FMDBZP_NTHR, in, out);
 }
 
 // ---------------------------------------------------------------------------
 // Column stage. One workgroup owns LB consecutive lags (FM-DBZP columns) of
-// one (PRN, column block) tile, with all ndop rows of each column in LDS.
+// one (PRN, column block) tile; lanes run across columns (COLFAST), so tile
+// loads and accumulator read-modify-writes are coalesced along code columns.
 // ---------------------------------------------------------------------------
 struct ColumnArgs {
     const float2* tile;      // [batch][nb][ndop][bs]
@@ -170,65 +205,75 @@ __device__ __forceinline__ void acc_cell(const ColumnArgs& A, int p, int row, in
     else *dst = *dst + v;
 }
 
-// Simple path: column FFT over ndop rows, |x|^2, fftshift fold
-// ((raw + ndop/2) mod ndop, src/plan.jl:528), drift, accumulate.
-// `_accumulate_fftshifted_power_(drift_)pilot!` and `_stream_power_column!`.
+struct TileColIn {
+    const float2* T;  // tile of (p, r), offset to lag0
+    int bs;
+    __device__ __forceinline__ float2 operator()(int t, int k) const { return T[(size_t)k * bs + t]; }
+};
+
+// Simple path output: |x|^2, fftshift fold ((raw + ndop/2) mod ndop,
+// src/plan.jl:528), drift, accumulate.
+struct SimpleAccOut {
+    const ColumnArgs* A;
+    int p, cp0, half;
+    __device__ __forceinline__ void operator()(int t, int w, float2 v) const
+    {
+        int row = w + half;
+        if (row >= A->ndop) row -= A->ndop;
+        acc_cell(*A, p, row, cp0 + t, 0, abs2_exact(v));
+    }
+};
+
+// `_accumulate_fftshifted_power_(drift_)pilot!` (src/noncoherent_integration.jl:482-592)
+// and `_stream_power_column!` (src/acquire.jl:335-385).
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_column_simple(ColumnArgs A)
 {
     extern __shared__ float2 lds[];
-    const int tid = threadIdx.x;
     const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
     const int lag0 = lb * A.LB;
     const int ncol = min(A.LB, A.bs - lag0);
-    const int ndop = A.ndop, bst = A.bstride;
-    float2* x = lds;
-    float2* y = lds + (size_t)A.LB * bst;
-    const float2* T = A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0;
-    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-        const int k = idx / ncol, c = idx - k * ncol;
-        x[c * bst + k] = T[(size_t)k * A.bs + c];
-    }
-    __syncthreads();
-    float2* res = fft_lds<false>(x, y, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR);
-    const int half = ndop / 2;
-    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-        const int w = idx / ncol, c = idx - w * ncol;
-        const float v = abs2_exact(res[c * bst + w]);
-        int row = w + half;
-        if (row >= ndop) row -= ndop;
-        acc_cell(A, p, row, r * A.bs + lag0 + c, 0, v);
-    }
+    const int bst = A.bstride;
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * A.ndop) * A.bs + lag0;
+    SimpleAccOut out{&A, p, r * A.bs + lag0, A.ndop / 2};
+    fft_run<false, true>(lds, lds + (size_t)A.LB * bst, A.pd, A.twD, ncol, bst, threadIdx.x, FMDBZP_NTHR,
+                         TileColIn{T, A.bs}, out);
 }
 
-// Fold raw FFT bins into the sorted stage rows the way the sign-search kernels
-// do (src/noncoherent_integration.jl:92-112, :238-294): raw w < half goes to
-// w + half, raw w >= half to w - half, first loop then second. For odd ndop
-// both raw 0 and raw ndop-1 land on row `half` and row ndop-1 is never
-// written; this reproduces that exactly by iterating destinations.
+// Power of each raw bin into an LDS float array (row-major per column).
 template <bool ROT>
-__device__ __forceinline__ void fold_into_stage(float* stage, const float2* res, int ncol, int ndop,
-                                                int bst, int use_max, int tid)
+struct PowerOut {
+    float* pw;
+    int ndop;
+    __device__ __forceinline__ void operator()(int t, int w, float2 z) const
+    {
+        pw[t * ndop + w] = ROT ? fmaf(z.x, z.x, __fmul_rn(z.y, z.y)) : abs2_exact(z);
+    }
+};
+
+// Fold raw-bin powers into the sorted stage rows the way the sign-search
+// kernels do (src/noncoherent_integration.jl:92-112, :238-294): raw w < half
+// goes to w + half, raw w >= half to w - half, first loop then second. For odd
+// ndop both raw 0 and raw ndop-1 land on row `half` and row ndop-1 is never
+// written; iterating destinations reproduces that exactly.
+__device__ __forceinline__ void fold_into_stage(float* stage, const float* pw, int ncol, int ndop, int use_max,
+                                                int tid)
 {
     const int half = ndop / 2;
     for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
         const int c = idx / ndop, dst = idx - c * ndop;
-        float cur = stage[c * ndop + dst];
-        // first loop: raw w in [0, half) -> dst = w + half
-        if (dst >= half && dst < 2 * half) {
-            const float2 z = res[c * bst + (dst - half)];
-            const float v = ROT ? fmaf(z.x, z.x, __fmul_rn(z.y, z.y)) : abs2_exact(z);
+        float cur = stage[idx];
+        if (dst >= half && dst < 2 * half) {  // first loop: raw dst - half
+            const float v = pw[c * ndop + dst - half];
             if (!use_max) cur = v;
             else if (v > cur) cur = v;
         }
-        // second loop: raw w in [half, ndop) -> dst = w - half
-        if (dst < ndop - half) {
-            const float2 z = res[c * bst + (dst + half)];
-            const float v = ROT ? fmaf(z.x, z.x, __fmul_rn(z.y, z.y)) : abs2_exact(z);
+        if (dst < ndop - half) {  // second loop: raw dst + half
+            const float v = pw[c * ndop + dst + half];
             if (!use_max) cur = v;
             else if (v > cur) cur = v;
         }
-        stage[c * ndop + dst] = cur;
+        stage[idx] = cur;
     }
 }
 
@@ -243,6 +288,37 @@ __device__ __forceinline__ void scatter_stage(const ColumnArgs& A, const float* 
     }
 }
 
+// Load the LB columns of the tile into LDS (x[c][k], stride bst), coalesced.
+__device__ __forceinline__ void load_columns(float2* x, const float2* T, int bs, int ndop, int ncol, int bst,
+                                             int tid)
+{
+    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
+        const int k = idx / ncol, c = idx - k * ncol;
+        x[c * bst + k] = T[(size_t)k * bs + c];
+    }
+}
+
+// Data-bit / bit-edge sign search input: the pattern-signed fold of the N_db
+// sub-blocks (rows ro + b*sub_len + n, circular) onto [0, sub_len), zero-padded.
+struct SignFoldIn {
+    const float2* x;
+    const float* sg;  // sign of each data bit for this pattern
+    int bst, ndop, ndb, sub_len, ro;
+    __device__ __forceinline__ float2 operator()(int t, int n) const
+    {
+        float2 v = make_float2(0.f, 0.f);
+        if (n < sub_len) {
+            for (int b = 0; b < ndb; b++) {
+                int src = (ro + b * sub_len + n) % ndop;
+                const float2 u = x[t * bst + src];
+                v.x += sg[b] * u.x;
+                v.y += sg[b] * u.y;
+            }
+        }
+        return v;
+    }
+};
+
 // Data-bit / bit-edge sign search (`_sign_search_column!`,
 // src/noncoherent_integration.jl:60-115, driven by `_fill_sign_stage_column!`
 // :654-686). For each edge offset and data-bit pattern d the reference sums
@@ -253,51 +329,53 @@ __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_column_sign(ColumnArgs A)
 {
     extern __shared__ float2 lds[];
+    __shared__ float sgn[32];
     const int tid = threadIdx.x;
     const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
     const int lag0 = lb * A.LB;
     const int ncol = min(A.LB, A.bs - lag0);
     const int ndop = A.ndop, bst = A.bstride;
     float2* x = lds;
-    float2* y = x + (size_t)A.LB * bst;
-    float2* z = y + (size_t)A.LB * bst;
-    float* stage = (float*)(z + (size_t)A.LB * bst);
-    const float2* T = A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0;
-    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-        const int k = idx / ncol, c = idx - k * ncol;
-        x[c * bst + k] = T[(size_t)k * A.bs + c];
-    }
+    float2* ya = x + (size_t)A.LB * bst;
+    float2* yb = ya + (size_t)A.LB * bst;
+    float* stage = (float*)(yb + (size_t)A.LB * bst);
+    float* pw = stage + (size_t)A.LB * ndop;
+    load_columns(x, A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0, A.bs, ndop, ncol, bst, tid);
     for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) stage[idx] = 0.f;
     const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
     const int sub_len = ndop / A.ndb;
     const int cpb = A.ncoh / A.ndb;
-    __syncthreads();
     for (int e = 0; e < A.E; e++) {
-        const int ro = e * A.edge_step;
         for (int d = 0; d < A.Q; d++) {
-            for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-                const int c = idx / ndop, n = idx - c * ndop;
-                float2 v = make_float2(0.f, 0.f);
-                if (n < sub_len) {
-                    for (int bb = 0; bb < A.ndb; bb++) {
-                        const float sg = pat[(size_t)d * A.ncoh + bb * cpb];
-                        int src = ro + bb * sub_len + n;
-                        src %= ndop;
-                        const float2 u = x[c * bst + src];
-                        v.x += sg * u.x;
-                        v.y += sg * u.y;
-                    }
-                }
-                y[c * bst + n] = v;
-            }
             __syncthreads();
-            float2* res = fft_lds<false>(y, z, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR);
-            fold_into_stage<false>(stage, res, ncol, ndop, bst, 1, tid);
+            if (tid < A.ndb) sgn[tid] = pat[(size_t)d * A.ncoh + tid * cpb];
             __syncthreads();
+            SignFoldIn in{x, sgn, bst, ndop, A.ndb, sub_len, e * A.edge_step};
+            fft_run<false, true>(ya, yb, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR, in, PowerOut<false>{pw, ndop});
+            __syncthreads();
+            fold_into_stage(stage, pw, ncol, ndop, 1, tid);
         }
     }
+    __syncthreads();
     scatter_stage(A, stage, p, r, lag0, ncol, 0, tid);
 }
+
+// Rotation search input: the column rotated by the per-column offset, times
+// the +-1 pattern of coherent period n / nb.
+struct RotIn {
+    const float2* x;
+    const float* pq;   // pattern q, one sign per coherent period
+    const int* eff;    // per-column row offset
+    int bst, ndop, nb;
+    __device__ __forceinline__ float2 operator()(int t, int n) const
+    {
+        int src = eff[t] + n;
+        if (src >= ndop) src -= ndop;
+        const float sg = pq[n / nb];
+        const float2 u = x[t * bst + src];
+        return make_float2(sg * u.x, sg * u.y);
+    }
+};
 
 // Secondary-code rotation search (`_rotation_column!`,
 // src/noncoherent_integration.jl:183-297). The reference combines N_coh
@@ -312,52 +390,46 @@ __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_column_rotation(ColumnArgs A)
 {
     extern __shared__ float2 lds[];
+    __shared__ float spat[256];
+    __shared__ int seff[64];
     const int tid = threadIdx.x;
     const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
     const int lag0 = lb * A.LB;
     const int ncol = min(A.LB, A.bs - lag0);
     const int ndop = A.ndop, bst = A.bstride;
     float2* x = lds;
-    float2* y = x + (size_t)A.LB * bst;
-    float2* z = y + (size_t)A.LB * bst;
-    float* stage = (float*)(z + (size_t)A.LB * bst);
-    const float2* T = A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0;
-    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-        const int k = idx / ncol, c = idx - k * ncol;
-        x[c * bst + k] = T[(size_t)k * A.bs + c];
-    }
+    float2* ya = x + (size_t)A.LB * bst;
+    float2* yb = ya + (size_t)A.LB * bst;
+    float* stage = (float*)(yb + (size_t)A.LB * bst);
+    float* pw = stage + (size_t)A.LB * ndop;
+    load_columns(x, A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0, A.bs, ndop, ncol, bst, tid);
     const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
     const int half_block = A.bs >> 1;
-    __syncthreads();
     for (int rot = 0; rot < A.R; rot++) {
         for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) stage[idx] = 0.f;
-        __syncthreads();
         for (int e = 0; e < A.E; e++) {
-            const int ro = e * A.edge_step;
+            __syncthreads();
+            // per-column alignment to the primary-code wrap (:212-215)
+            for (int c = tid; c < ncol; c += FMDBZP_NTHR) {
+                const int pc = r * A.bs + lag0 + c;
+                const int wrap = ((A.nb - pc / A.bs) % A.nb) * A.bs + (pc % A.bs);
+                const int cro = ((wrap + half_block) / A.bs) % A.nb;
+                seff[c] = (e * A.edge_step + cro) % ndop;
+            }
             for (int d = 0; d < A.ndc; d++) {
                 const int q = rot * A.ndc + d;
-                const float* pq = pat + (size_t)q * A.ncoh;
-                for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-                    const int c = idx / ndop, n = idx - c * ndop;
-                    // per-column alignment to the primary-code wrap (:212-215)
-                    const int pc = r * A.bs + lag0 + c;
-                    const int wrap = ((A.nb - pc / A.bs) % A.nb) * A.bs + (pc % A.bs);
-                    const int cro = ((wrap + half_block) / A.bs) % A.nb;
-                    const int eff = (ro + cro) % ndop;
-                    int src = eff + n;
-                    if (src >= ndop) src -= ndop;
-                    const float sg = pq[n / A.nb];
-                    const float2 u = x[c * bst + src];
-                    y[c * bst + n] = make_float2(sg * u.x, sg * u.y);
-                }
                 __syncthreads();
-                float2* res = fft_lds<false>(y, z, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR);
-                fold_into_stage<true>(stage, res, ncol, ndop, bst, A.use_max, tid);
+                for (int i = tid; i < A.ncoh; i += FMDBZP_NTHR) spat[i] = pat[(size_t)q * A.ncoh + i];
                 __syncthreads();
+                RotIn in{x, spat, seff, bst, ndop, A.nb};
+                fft_run<false, true>(ya, yb, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR, in, PowerOut<true>{pw, ndop});
+                __syncthreads();
+                fold_into_stage(stage, pw, ncol, ndop, A.use_max, tid);
             }
         }
-        scatter_stage(A, stage, p, r, lag0, ncol, rot, tid);
         __syncthreads();
+        scatter_stage(A, stage, p, r, lag0, ncol, rot, tid);
+        __syncthreads();  // the next rotation zeroes the stage this scatter reads
     }
 }
 

@@ -51,7 +51,12 @@ enum fmdbzp_eltype {
     FMDBZP_CI8 = 3,      /* interleaved int8 re,im (Julia Complex{Int8}) */
     FMDBZP_CI16 = 4,     /* interleaved int16 re,im (Julia Complex{Int16}) */
     FMDBZP_F32 = 5,      /* real float (Julia Float32) */
-    FMDBZP_F64 = 6       /* real double (Julia Float64) */
+    FMDBZP_F64 = 6,      /* real double (Julia Float64) */
+    FMDBZP_C2BIT = 7     /* packed 2-bit complex SDR samples: byte i holds samples
+                            2i (low nibble) and 2i+1 (high nibble); a nibble is
+                            I in bits 0-1 and Q in bits 2-3, each code c mapping to
+                            (c & 1 ? 3 : 1) * (c & 2 ? -1 : +1), i.e. {+1, +3, -1, -3}.
+                            The reference sees the same values as Complex{Int8}. */
 };
 
 /* Search path, as selected by the reference (src/acquire.jl:250-253,
@@ -170,6 +175,35 @@ int fmdbzp_acquire(fmdbzp_plan* plan,
                    uint32_t flags,
                    fmdbzp_prn_stats* out_stats,
                    float* const* out_surfaces);
+
+/*
+ * One request of a batch: the arguments of fmdbzp_acquire. `nprn` PRN
+ * indices of `plan` searched over `signal`.
+ */
+typedef struct fmdbzp_request {
+    fmdbzp_plan* plan;
+    const void* signal;
+    int32_t eltype;
+    int32_t nprn;
+    int64_t n_samples;
+    double interm_freq_hz;
+    const int32_t* prn_index;
+    uint32_t flags;
+    uint32_t reserved;
+    fmdbzp_prn_stats* out_stats;
+    float* const* out_surfaces;
+} fmdbzp_request;
+
+/*
+ * Run `nreqs` searches of possibly different plans (geometry, system, sample
+ * rate, Doppler span, PRN count, N_nc) as one submission: every request is
+ * enqueued on its plan's stream with no host synchronisation in between, so
+ * requests on different plans run concurrently on the device, and the call
+ * returns when all have finished. No counterpart in the reference, which runs
+ * one `acquire!` per plan (the batched uneven mix is a sequence of `acquire!`
+ * calls there). A plan listed twice runs its requests in order.
+ */
+int fmdbzp_acquire_batch(const fmdbzp_request* reqs, int32_t nreqs);
 
 /* Release every device allocation of the plan. NULL is accepted. */
 int fmdbzp_plan_destroy(fmdbzp_plan* plan);

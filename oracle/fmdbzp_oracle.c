@@ -181,6 +181,16 @@ static inline void load_sample(const void* sig, int eltype, int64_t i, float* re
     case FMDBZP_CI16: { const int16_t* p = (const int16_t*)sig; *re = (float)p[2 * i]; *im = (float)p[2 * i + 1]; break; }
     case FMDBZP_F32: { const float* p = (const float*)sig; *re = p[i]; *im = 0.0f; break; }
     case FMDBZP_F64: { const double* p = (const double*)sig; *re = (float)p[i]; *im = 0.0f; break; }
+    case FMDBZP_C2BIT: {
+        /* two complex samples per byte, low nibble first; 2-bit codes
+         * {0,1,2,3} -> {+1,+3,-1,-3} for I (bits 0-1) and Q (bits 2-3) */
+        unsigned b = ((const uint8_t*)sig)[i >> 1];
+        unsigned nib = (i & 1) ? (b >> 4) : (b & 15u);
+        unsigned ci = nib & 3u, cq = (nib >> 2) & 3u;
+        *re = (ci & 1u ? 3.0f : 1.0f) * (ci & 2u ? -1.0f : 1.0f);
+        *im = (cq & 1u ? 3.0f : 1.0f) * (cq & 2u ? -1.0f : 1.0f);
+        break;
+    }
     default: *re = 0.0f; *im = 0.0f;
     }
 }

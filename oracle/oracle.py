@@ -84,10 +84,12 @@ def lib(bits: int = 32) -> ctypes.CDLL:
     return L
 
 
-ELTYPE = {"cf32": 0, "cf64": 1, "i16": 2, "ci8": 3, "ci16": 4, "f32": 5, "f64": 6}
+ELTYPE = {"cf32": 0, "cf64": 1, "i16": 2, "ci8": 3, "ci16": 4, "f32": 5, "f64": 6, "c2bit": 7}
 
 
-def _eltype(a: np.ndarray):
+def _eltype(a):
+    if hasattr(a, "packed") and hasattr(a, "n"):
+        return ELTYPE["c2bit"], a.n
     if a.dtype == np.complex64:
         return ELTYPE["cf32"], a.shape[0]
     if a.dtype == np.complex128:
@@ -103,6 +105,15 @@ def _eltype(a: np.ndarray):
     if a.dtype == np.float64:
         return ELTYPE["f64"], a.shape[0]
     raise TypeError(f"unsupported signal dtype {a.dtype}")
+
+
+def _sig(signal):
+    """(contiguous array holding the samples, eltype id, sample count)."""
+    if hasattr(signal, "packed") and hasattr(signal, "n"):
+        return np.ascontiguousarray(signal.packed), ELTYPE["c2bit"], signal.n
+    a = np.ascontiguousarray(signal)
+    et, n = _eltype(a)
+    return a, et, n
 
 
 def desc_struct(plan):
@@ -142,8 +153,7 @@ def acquire(plan, signal, prn_index, *, if_hz=0.0, subsample_interpolation=False
         pats = np.ones(1, dtype=np.float32)
     shifts = np.ascontiguousarray(plan.code_drift_shifts, dtype=np.int32)
     dop = np.ascontiguousarray(plan.doppler_freqs, dtype=np.float64)
-    sig = np.ascontiguousarray(signal)
-    et, n = _eltype(sig)
+    sig, et, n = _sig(signal)
     idx = np.ascontiguousarray(prn_index, dtype=np.int32)
     nreq = len(idx)
     res = (OResult * nreq)()
@@ -207,8 +217,7 @@ def sign_patterns(sec, ndb, R, ncoh, use_sec):
 
 def signal_block_ffts(plan, signal, seg0=0, if_hz=0.0, bits=32):
     d = desc_struct(plan)
-    sig = np.ascontiguousarray(signal)
-    et, _ = _eltype(sig)
+    sig, et, _ = _sig(signal)
     nrow = plan.num_coherently_integrated_code_periods * plan.num_blocks
     out = np.zeros(nrow * plan.fft_size * 2)
     lib(bits).oracle_signal_block_ffts(ctypes.byref(d), sig.ctypes.data, et, int(seg0), float(if_hz), out.ctypes.data)
@@ -217,8 +226,7 @@ def signal_block_ffts(plan, signal, seg0=0, if_hz=0.0, bits=32):
 
 def coherent_matrix(plan, code, signal, seg0=0, if_hz=0.0, bits=32):
     d = desc_struct(plan)
-    sig = np.ascontiguousarray(signal)
-    et, _ = _eltype(sig)
+    sig, et, _ = _sig(signal)
     c = np.ascontiguousarray(code, dtype=np.int8)
     out = np.zeros(plan.num_doppler_bins * plan.samples_per_code * 2)
     lib(bits).oracle_coherent_matrix(ctypes.byref(d), c.ctypes.data, sig.ctypes.data, et, int(seg0), float(if_hz),
@@ -228,16 +236,14 @@ def coherent_matrix(plan, code, signal, seg0=0, if_hz=0.0, bits=32):
 
 
 def downconvert(signal, seg0, seglen, if_hz, fs, bits=32):
-    sig = np.ascontiguousarray(signal)
-    et, _ = _eltype(sig)
+    sig, et, _ = _sig(signal)
     out = np.zeros(seglen * 2)
     lib(bits).oracle_downconvert(sig.ctypes.data, et, int(seg0), int(seglen), float(if_hz), float(fs), out.ctypes.data)
     return out.view(np.complex128)
 
 
 def secondary_phase(signal, spc, L, fs, if_hz, doppler_hz, code, sec):
-    sig = np.ascontiguousarray(signal)
-    et, _ = _eltype(sig)
+    sig, et, _ = _sig(signal)
     c = np.ascontiguousarray(code, dtype=np.int8)
     s = np.ascontiguousarray(sec, dtype=np.int8)
     return lib(32).oracle_secondary_phase(sig.ctypes.data, et, int(spc), int(L), float(fs), float(if_hz),

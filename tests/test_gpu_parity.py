@@ -238,3 +238,63 @@ def test_plan_reuse_and_errors():
             acquire_(plan, g1["signal"][:1000], [1])
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("kind", ["ci8", "2bit"])
+def test_quantized_complex_if_input(kind):
+    """Complex{Int8} and packed 2-bit IF samples (16.368 MHz, IF 4.092 MHz), N_coh 10, N_nc 2."""
+    from acquisition.signals import l1ca_quantized_scene
+
+    sc = l1ca_quantized_scene(kind=kind, n_ms=20, npresent=3, cn0_range=(44.0, 46.0))
+    prns = sc.present[:2] + [p for p in range(1, 33) if p not in sc.present][:1]
+    plan, res, ores = run_case(sc.system, sc.sampling_freq, prns, sc.signal, if_hz=sc.interm_freq, interp=True,
+                               label=kind, num_coherently_integrated_code_periods=10,
+                               num_noncoherent_accumulations=2)
+    assert is_detected(res[0]) and is_detected(res[1])
+
+
+def test_e1like_rotation_boc_reduced():
+    """E1-like BOC(1,1) pilot with a 25-chip secondary code at 2.046 MHz,
+    N_coh 25 (R 25), +-1 kHz: the rotation search at a size the oracle finishes."""
+    from acquisition import E1Like
+    from acquisition.signals import e1like_scene
+
+    sysm = E1Like()
+    sc = e1like_scene(n_periods=25, nnc=1, fs=2.046e6, nprn=4, npresent=2, cn0_range=(46.0, 48.0), system=sysm)
+    p0 = sc.present[0]
+    prns = [p0, 1 if p0 != 1 else 4]
+    plan, res, ores = run_case(sysm, sc.sampling_freq, prns, sc.signal, interp=True, label="e1like",
+                               min_doppler_coverage=1000.0, num_coherently_integrated_code_periods=25)
+    assert plan.path == 2 and plan.num_secondary_rotations == 25
+    assert is_detected(res[0])
+    assert res[0].secondary_code_phase == sc.truth[p0]["secondary_phase"]
+
+
+def test_batch_of_uneven_requests_matches_single_calls():
+    """fmdbzp_acquire_batch over three plans of different geometry (L1 C/A at 5 and
+    16.368 MHz, L5I rotation search) returns exactly the single-call results."""
+    from acquisition import acquire_batch
+    from acquisition.signals import l1ca_5mhz_scene
+
+    sc1 = l1ca_5mhz_scene(seed=91)
+    sc2 = l1ca_if_int16_scene(seed=92, n_ms=20, npresent=2, cn0_range=(44.0, 45.0))
+    sig3 = l5i_clean(fs=12e6, r0=5, cp=321.0)
+    p1 = plan_acquire(sc1.system, sc1.sampling_freq, sc1.prns)
+    p2 = plan_acquire(sc2.system, sc2.sampling_freq, sc2.prns[:8], num_coherently_integrated_code_periods=10,
+                      num_noncoherent_accumulations=2, bit_edge_search_steps=2)
+    p3 = plan_acquire(GPSL5I(), 12e6, [1, 2], num_coherently_integrated_code_periods=10)
+    try:
+        reqs = [(p1, sc1.signal, sc1.prns), (p2, sc2.signal, sc2.prns[:8], dict(interm_freq=sc2.interm_freq)),
+                (p3, sig3, [1, 2], dict(subsample_interpolation=True)), (p1, sc1.signal, [3, 4])]
+        batch = acquire_batch(reqs)
+        single = [acquire_(p1, sc1.signal, sc1.prns), acquire_(p2, sc2.signal, sc2.prns[:8], interm_freq=sc2.interm_freq),
+                  acquire_(p3, sig3, [1, 2], subsample_interpolation=True), acquire_(p1, sc1.signal, [3, 4])]
+        for rb, rs in zip(batch, single):
+            assert [r.engine_stats for r in rb] == [r.engine_stats for r in rs]
+            assert [(r.code_phase, r.carrier_doppler, r.secondary_code_phase) for r in rb] == \
+                [(r.code_phase, r.carrier_doppler, r.secondary_code_phase) for r in rs]
+        assert batch[2][0].secondary_code_phase == 5
+    finally:
+        p1.close()
+        p2.close()
+        p3.close()

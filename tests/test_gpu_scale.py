@@ -1,0 +1,125 @@
+"""GPU checks at the measured workload's full size.
+
+- Parity with the oracle on the benchmark workload itself (GPS L1 C/A,
+  16.368 MHz int16 IF, 80 ms, 32 PRNs, N_coh 10, bit-edge steps 10, N_nc 8).
+- Size-independent properties that hold bit-exactly:
+  * scaling the int16 samples by 2 multiplies every power by exactly 4
+    (power-of-two scaling is exact through every FP32 operation), so peaks,
+    noise x4 and peak-to-noise ratios must match bit for bit;
+  * the PRN request order and the PRN batch size do not change any result.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from acquisition import acquire_, cfar_threshold, get_num_cells, is_detected, plan_acquire
+from acquisition.signals import l1ca_if_int16_scene
+from parity import check_prn, detection_consistent
+
+pytestmark = pytest.mark.gpu
+
+PLAN_KW = dict(num_coherently_integrated_code_periods=10, bit_edge_search_steps=10, num_noncoherent_accumulations=8)
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return l1ca_if_int16_scene(seed=0x5EED0002)
+
+
+def stats_tuple(r):
+    es = r.engine_stats
+    return (es["peak"], es["row"], es["col"], es["noise"], tuple(es["nbr"]))
+
+
+def test_full_workload_parity(scene):
+    sc = scene
+    prns = sc.prns
+    plan = plan_acquire(sc.system, sc.sampling_freq, prns, **PLAN_KW)
+    try:
+        res = acquire_(plan, sc.signal, prns, interm_freq=sc.interm_freq, subsample_interpolation=True)
+        ores, _ = oracle.acquire(plan, sc.signal, list(range(32)), if_hz=sc.interm_freq,
+                                 subsample_interpolation=True, nthreads=16)
+        det_g, det_o = [], []
+        for j, p in enumerate(prns):
+            check_prn(res[j], ores[j], None, None, interp=True, label=f"c2 prn {p}")
+            thr = cfar_threshold(0.01, get_num_cells(res[j]), num_noncoherent_integrations=8)
+            assert detection_consistent(res[j], ores[j], thr)
+            if is_detected(res[j]):
+                det_g.append(p)
+            if ores[j]["ptn"] > thr:
+                det_o.append(p)
+        print("detected gpu", det_g, "oracle", det_o, "present", sc.present)
+        assert det_g == det_o
+        assert set(det_g) <= set(sc.present)
+    finally:
+        plan.close()
+
+
+def test_full_workload_surfaces_present_prns(scene):
+    sc = scene
+    sub = sc.present[:3]
+    plan = plan_acquire(sc.system, sc.sampling_freq, sub, **PLAN_KW)
+    try:
+        res = acquire_(plan, sc.signal, sub, interm_freq=sc.interm_freq, store_power_bins=True)
+        ores, osurf = oracle.acquire(plan, sc.signal, list(range(len(sub))), if_hz=sc.interm_freq, surfaces=True,
+                                     nthreads=16)
+        for j, p in enumerate(sub):
+            print(check_prn(res[j], ores[j], osurf[j], res[j].power_bins, label=f"c2-surf prn {p}"))
+    finally:
+        plan.close()
+
+
+def test_power_of_two_scaling_is_exact(scene):
+    sc = scene
+    x2 = (sc.signal.astype(np.int32) * 2).clip(-32768, 32767).astype(np.int16)
+    assert np.array_equal(x2.astype(np.int32), sc.signal.astype(np.int32) * 2)
+    prns = sc.prns[:8]
+    plan = plan_acquire(sc.system, sc.sampling_freq, prns, **PLAN_KW)
+    try:
+        a = acquire_(plan, sc.signal, prns, interm_freq=sc.interm_freq)
+        b = acquire_(plan, x2, prns, interm_freq=sc.interm_freq)
+        for ra, rb in zip(a, b):
+            sa, sb = ra.engine_stats, rb.engine_stats
+            assert (sa["row"], sa["col"]) == (sb["row"], sb["col"])
+            assert np.float32(sb["peak"]) == np.float32(4) * np.float32(sa["peak"])
+            assert np.float32(sb["noise"]) == np.float32(4) * np.float32(sa["noise"])
+            assert ra.peak_to_noise_ratio == rb.peak_to_noise_ratio
+    finally:
+        plan.close()
+
+
+def test_prn_order_and_batch_size_do_not_change_results(scene):
+    sc = scene
+    prns = sc.prns
+    plan = plan_acquire(sc.system, sc.sampling_freq, prns, **PLAN_KW)
+    try:
+        a = {r.prn: stats_tuple(r) for r in acquire_(plan, sc.signal, prns, interm_freq=sc.interm_freq)}
+        rev = list(reversed(prns))
+        b = {r.prn: stats_tuple(r) for r in acquire_(plan, sc.signal, rev, interm_freq=sc.interm_freq)}
+        assert a == b
+    finally:
+        plan.close()
+    # a fresh process with a tiny workspace budget forces one PRN per batch
+    code = f"""
+import sys, json
+sys.path[:0] = [{os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'acquisition.jl_amd')!r}]
+from acquisition import acquire_, plan_acquire
+from acquisition.signals import l1ca_if_int16_scene
+sc = l1ca_if_int16_scene(seed=0x5EED0002)
+plan = plan_acquire(sc.system, sc.sampling_freq, sc.prns, **{PLAN_KW!r})
+out = {{r.prn: [r.engine_stats["peak"], r.engine_stats["row"], r.engine_stats["col"], r.engine_stats["noise"],
+              r.engine_stats["nbr"]] for r in acquire_(plan, sc.signal, sc.prns, interm_freq=sc.interm_freq)}}
+print(json.dumps(out))
+"""
+    env = dict(os.environ, FMDBZP_WORKSPACE_GB="0.03")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=170, env=env)
+    assert r.returncode == 0, r.stderr
+    import json
+
+    c = json.loads(r.stdout.strip().splitlines()[-1])
+    for p, v in c.items():
+        assert (v[0], v[1], v[2], v[3], tuple(v[4])) == a[int(p)]
