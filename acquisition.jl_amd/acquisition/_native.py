@@ -43,7 +43,8 @@ class PlanDesc(ctypes.Structure):
         ("noise_estimator", ctypes.c_int32),
         ("samples_per_chip", ctypes.c_int32),
         ("precision", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("options", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
         ("sampling_freq_hz", ctypes.c_double),
     ]
 

@@ -27,6 +27,7 @@ class ArgumentError(ValueError):
 BATCH_FFT_THRESHOLD = 320
 
 PATH_SIMPLE, PATH_SIGN, PATH_ROTATION = 0, 1, 2
+OPT_LITERAL_EDGES = 0x1  # FMDBZP_OPT_LITERAL_EDGES
 
 
 def nextprod(factors: Sequence[int], n: int) -> int:
@@ -309,12 +310,20 @@ def plan_acquire(system: GNSSSystem, sampling_freq: float, prns: Sequence[int], 
                  noise_estimator: AbstractNoiseEstimator = OppositeRowNoiseEstimator(),
                  use_secondary_code: bool = True,
                  max_secondary_code_rotations: int = 32,
-                 fft_flag=None) -> AcquisitionPlan:
+                 fft_flag=None,
+                 literal_edges: bool = False) -> AcquisitionPlan:
     """Pre-compute an FM-DBZP acquisition plan (src/plan.jl:312-324).
 
     Keyword arguments and validation errors are the reference's; frequencies
     are plain Hz floats. `fft_flag` (FFTW planning) has no effect: the engine's
     transforms are fixed kernels. The plan owns device memory until `close()`.
+
+    `literal_edges` (engine option, no reference counterpart): on the sign
+    path with one data bit, the bit-edge offsets only rotate the column
+    circularly, so every edge gives the same |FFT|^2 and the engine evaluates
+    one of them. True evaluates all `bit_edge_search_steps` offsets as the
+    reference does (src/noncoherent_integration.jl:670-685); results agree to
+    rounding.
     """
     from . import _native
 
@@ -326,6 +335,7 @@ def plan_acquire(system: GNSSSystem, sampling_freq: float, prns: Sequence[int], 
                       noise_estimator=noise_estimator,
                       use_secondary_code=use_secondary_code,
                       max_secondary_code_rotations=max_secondary_code_rotations)
+    plan.literal_edges = bool(literal_edges)
     plan.native = _native.NativePlan.create(plan)
     return plan
 
@@ -349,6 +359,7 @@ def plan_desc_fields(plan: AcquisitionPlan) -> dict:
         noise_estimator=plan.noise_estimator_id,
         samples_per_chip=plan.samples_per_chip,
         precision=0,
+        options=OPT_LITERAL_EDGES if getattr(plan, "literal_edges", False) else 0,
         sampling_freq_hz=plan.sampling_freq,
     )
 

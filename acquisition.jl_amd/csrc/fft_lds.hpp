@@ -12,8 +12,8 @@
 // Ns = product of earlier radices maps butterfly j (of n/R) to inputs
 // j + r n/R and outputs (j - j mod Ns) R + (j mod Ns) + r Ns, with twiddle
 // exp(-2 pi i (j mod Ns) r / (Ns R)) read from one FP64-computed table of n
-// entries. Radices 2, 3, 4, 5, 8 have unrolled butterflies; any other prime
-// uses a direct R-point DFT read straight from LDS.
+// entries. Radices 2, 3, 4, 5, 8, 16 have unrolled butterflies; any other
+// prime uses a direct R-point DFT read straight from LDS.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -138,6 +138,49 @@ __device__ __forceinline__ void bfly5(float2* v)
     v[3] = csubf(a2, mb2);
 }
 
+// x * w for a constant twiddle w = (wr, wi) of the forward transform; the
+// inverse transform uses conj(w).
+template <bool INV>
+__device__ __forceinline__ float2 cmul_const(float2 x, float wr, float wi)
+{
+    if (INV) wi = -wi;
+    return make_float2(x.x * wr - x.y * wi, x.x * wi + x.y * wr);
+}
+
+// 16-point DFT as 4 x 4: radix-4 DFTs over q of v[i + 4q], internal twiddles
+// W16^(i p), radix-4 DFTs over i; output k = p + 4 s.
+template <bool INV>
+__device__ __forceinline__ void bfly16(float2* v)
+{
+    const float c = 0.92387953251128675613f;  // cos(pi/8)
+    const float s = 0.38268343236508977173f;  // sin(pi/8)
+    const float h = 0.70710678118654752440f;  // cos(pi/4)
+    float2 A[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) A[i][q] = v[i + 4 * q];
+        bfly4<INV>(A[i]);
+    }
+    // W16^m = exp(-2 pi i m / 16), m = i p
+    A[1][1] = cmul_const<INV>(A[1][1], c, -s);
+    A[1][2] = cmul_const<INV>(A[1][2], h, -h);
+    A[1][3] = cmul_const<INV>(A[1][3], s, -c);
+    A[2][1] = cmul_const<INV>(A[2][1], h, -h);
+    A[2][2] = mul_mi<INV>(A[2][2]);
+    A[2][3] = cmul_const<INV>(A[2][3], -h, -h);
+    A[3][1] = cmul_const<INV>(A[3][1], s, -c);
+    A[3][2] = cmul_const<INV>(A[3][2], -h, -h);
+    A[3][3] = cmul_const<INV>(A[3][3], -c, s);
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        float2 B[4] = {A[0][p], A[1][p], A[2][p], A[3][p]};
+        bfly4<INV>(B);
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[p + 4 * q] = B[q];
+    }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void bfly(float2* v)
 {
@@ -145,7 +188,8 @@ __device__ __forceinline__ void bfly(float2* v)
     else if constexpr (R == 3) bfly3<INV>(v);
     else if constexpr (R == 4) bfly4<INV>(v);
     else if constexpr (R == 5) bfly5<INV>(v);
-    else bfly8<INV>(v);
+    else if constexpr (R == 8) bfly8<INV>(v);
+    else bfly16<INV>(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -159,16 +203,50 @@ __device__ __forceinline__ void bfly(float2* v)
 // lanes work on consecutive transforms at the same element, which coalesces
 // column-strided global data) or element fastest (one long transform).
 // ---------------------------------------------------------------------------
+// LDS element index with one pad slot every 16 elements: the strided
+// butterfly writes of the early Stockham stages then spread over the banks
+// (worst case 3-way instead of 16-way for F = 2048). A transform of n points
+// needs fft_lds_span(n) slots.
+__host__ __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+__host__ __device__ __forceinline__ int fft_lds_span(int n) { return n + (n >> 4) + 1; }
+
+// Padding applies to element-fastest (single long) transforms. With the
+// transform index fastest (COLFAST) consecutive lanes are bst apart, and an
+// odd bst already spreads them over the banks, so no padding is used.
+template <bool PAD>
+__host__ __device__ __forceinline__ int lds_pad_if(int i) { return PAD ? lds_pad(i) : i; }
+
+template <bool PAD>
 struct LdsIn {
     const float2* buf;
     int bst;
-    __device__ __forceinline__ float2 operator()(int t, int i) const { return buf[t * bst + i]; }
+    __device__ __forceinline__ float2 operator()(int t, int i) const { return buf[t * bst + lds_pad_if<PAD>(i)]; }
 };
+template <bool PAD>
 struct LdsOut {
     float2* buf;
     int bst;
-    __device__ __forceinline__ void operator()(int t, int i, float2 v) const { buf[t * bst + i] = v; }
+    __device__ __forceinline__ void operator()(int t, int i, float2 v) const
+    {
+        buf[t * bst + lds_pad_if<PAD>(i)] = v;
+    }
 };
+
+// i mod d for the Stockham sub-transform index: a mask when d is a power of
+// two (uniform branch), else a remainder.
+__device__ __forceinline__ int mod_ns(int j, int d) { return (d & (d - 1)) == 0 ? (j & (d - 1)) : j % d; }
+
+// q = x / d, r = x - q d for 0 <= x < 2^24 via a float reciprocal and one
+// correction step (inv_d = 1.0f / d).
+__device__ __forceinline__ int fdivmod(int x, int d, float inv_d, int* r)
+{
+    int q = (int)((float)x * inv_d);
+    int rr = x - q * d;
+    if (rr < 0) { q -= 1; rr += d; }
+    else if (rr >= d) { q += 1; rr -= d; }
+    *r = rr;
+    return q;
+}
 
 template <int R, bool INV, bool COLFAST, class In, class Out>
 __device__ __forceinline__ void stage_io(const In& in, const Out& out, int n, int Ns, int nbatch,
@@ -177,11 +255,14 @@ __device__ __forceinline__ void stage_io(const In& in, const Out& out, int n, in
     const int m = n / R;
     const int tstride = n / (Ns * R);
     const int total = nbatch * m;
+    const float inv_b = 1.0f / (float)(COLFAST ? nbatch : m);
+#pragma unroll 1
     for (int idx = tid; idx < total; idx += nthr) {
         int t, j;
-        if (COLFAST) { j = idx / nbatch; t = idx - j * nbatch; }
-        else { t = idx / m; j = idx - t * m; }
-        const int k = j % Ns;
+        if (COLFAST) j = fdivmod(idx, nbatch, inv_b, &t);
+        else if (nbatch == 1) { t = 0; j = idx; }
+        else t = fdivmod(idx, m, inv_b, &j);
+        const int k = mod_ns(j, Ns);
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = in(t, j + r * m);
@@ -233,6 +314,9 @@ __device__ __forceinline__ void stage_any(int R, const In& in, const Out& out, i
     case 4: stage_io<4, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
     case 5: stage_io<5, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
     case 8: stage_io<8, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
+    case 16:  // planned only for element-fast transforms
+        if constexpr (!COLFAST) stage_io<16, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr);
+        break;
     default: stage_io_generic<INV, COLFAST>(in, out, n, R, Ns, nbatch, tw, tid, nthr); break;
     }
 }
@@ -253,18 +337,124 @@ __device__ void fft_run(float2* a, float2* b, const FftPlanDev& p, const float2*
         stage_any<INV, COLFAST>(p.radix[0], gin, gout, n, 1, nbatch, tw, tid, nthr);
         return;
     }
-    stage_any<INV, COLFAST>(p.radix[0], gin, LdsOut{a, bst}, n, 1, nbatch, tw, tid, nthr);
+    stage_any<INV, COLFAST>(p.radix[0], gin, LdsOut<!COLFAST>{a, bst}, n, 1, nbatch, tw, tid, nthr);
     __syncthreads();
     int Ns = p.radix[0];
     float2* src = a;
     float2* dst = b;
     for (int s = 1; s < p.nst - 1; s++) {
-        stage_any<INV, COLFAST>(p.radix[s], LdsIn{src, bst}, LdsOut{dst, bst}, n, Ns, nbatch, tw, tid, nthr);
+        stage_any<INV, COLFAST>(p.radix[s], LdsIn<!COLFAST>{src, bst}, LdsOut<!COLFAST>{dst, bst}, n, Ns, nbatch, tw,
+                                tid, nthr);
         __syncthreads();
         float2* t = src;
         src = dst;
         dst = t;
         Ns *= p.radix[s];
     }
-    stage_any<INV, COLFAST>(p.radix[p.nst - 1], LdsIn{src, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
+    stage_any<INV, COLFAST>(p.radix[p.nst - 1], LdsIn<!COLFAST>{src, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
+}
+
+// ---------------------------------------------------------------------------
+// Single-buffer variant: intermediate stages run in place on one LDS buffer.
+// Each thread reads all of its butterflies' inputs into registers, the block
+// synchronises, then every thread writes its outputs. That halves the LDS a
+// transform occupies (more columns per workgroup) at the price of one extra
+// barrier per intermediate stage. A thread holds at most MAXV complex values
+// per stage, so nbatch * n <= MAXV * nthr is required (checked on the host).
+// ---------------------------------------------------------------------------
+template <int R, bool INV, bool COLFAST, int MAXV>
+__device__ __noinline__ void stage_inplace(float2* a, int bst, int n, int Ns, int nbatch,
+                                              const float2* __restrict__ tw, int tid, int nthr)
+{
+    constexpr int MB = (MAXV + R - 1) / R;
+    const int m = n / R;
+    const int tstride = n / (Ns * R);
+    const int total = nbatch * m;
+    const float inv_b = 1.0f / (float)(COLFAST ? nbatch : m);
+    float2 v[MB][R];
+    int ot[MB], ob[MB];  // transform offset (-1: no butterfly) and output base
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int idx = tid + b * nthr;
+        ot[b] = -1;
+        ob[b] = 0;
+        if (idx < total) {
+            int t, j;
+            if (COLFAST) j = fdivmod(idx, nbatch, inv_b, &t);
+            else if (nbatch == 1) { t = 0; j = idx; }
+            else t = fdivmod(idx, m, inv_b, &j);
+            const int k = mod_ns(j, Ns);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = a[t * bst + lds_pad_if<!COLFAST>(j + r * m)];
+            if (Ns > 1) {
+                const int ts = k * tstride;
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b][r] = cmulf(v[b][r], twid<INV>(tw, r * ts));
+            }
+            bfly<R, INV>(v[b]);
+            ot[b] = t * bst;
+            ob[b] = (j - k) * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        if (ot[b] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) a[ot[b] + lds_pad_if<!COLFAST>(ob[b] + r * Ns)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+inline bool fft_inplace_ok(const FftPlanDev& p)
+{
+    for (int s = 1; s + 1 < p.nst; s++) {
+        const int r = p.radix[s];
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8 && r != 16) return false;
+    }
+    return true;
+}
+
+template <bool INV, bool COLFAST, int MAXV>
+__device__ __forceinline__ void stage_inplace_any(int R, float2* a, int bst, int n, int Ns, int nbatch,
+                                                  const float2* __restrict__ tw, int tid, int nthr)
+{
+    switch (R) {
+    case 2: stage_inplace<2, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
+    case 3: stage_inplace<3, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
+    case 4: stage_inplace<4, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
+    case 5: stage_inplace<5, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
+    case 8: stage_inplace<8, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
+    default:  // 16, planned only for element-fast transforms
+        if constexpr (!COLFAST) stage_inplace<16, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr);
+        break;
+    }
+}
+
+// Whole transform through gin/gout with one LDS buffer `a` (stride bst).
+// Same contract as fft_run; additionally nbatch * n <= MAXV * nthr, and the
+// intermediate stages (all but the first and last) have radix 2, 3, 4, 5, 8
+// or 16 (fft_inplace_ok).
+template <bool INV, bool COLFAST, int MAXV, class GIn, class GOut>
+__device__ void fft_run_1buf(float2* a, const FftPlanDev& p, const float2* __restrict__ tw, int nbatch, int bst,
+                             int tid, int nthr, const GIn& gin, const GOut& gout)
+{
+    const int n = p.n;
+    if (p.nst == 0) {
+        for (int t = tid; t < nbatch; t += nthr) gout(t, 0, gin(t, 0));
+        return;
+    }
+    if (p.nst == 1) {
+        stage_any<INV, COLFAST>(p.radix[0], gin, gout, n, 1, nbatch, tw, tid, nthr);
+        return;
+    }
+    stage_any<INV, COLFAST>(p.radix[0], gin, LdsOut<!COLFAST>{a, bst}, n, 1, nbatch, tw, tid, nthr);
+    __syncthreads();
+    int Ns = p.radix[0];
+    for (int s = 1; s < p.nst - 1; s++) {
+        stage_inplace_any<INV, COLFAST, MAXV>(p.radix[s], a, bst, n, Ns, nbatch, tw, tid, nthr);
+        Ns *= p.radix[s];
+    }
+    stage_any<INV, COLFAST>(p.radix[p.nst - 1], LdsIn<!COLFAST>{a, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
 }

@@ -77,7 +77,9 @@ std::vector<float2> twiddles(int n)
 }
 
 // Radix schedule: 8s, then one 4 or 2, then 5s, 3s, 7s and larger primes.
-int make_fft_plan(int n, FftPlanDev* p)
+// Radix sequence of an n-point transform: 16s (when allowed), then 8, 4, 2,
+// 5, 3, then any larger prime.
+int make_fft_plan(int n, FftPlanDev* p, bool radix16)
 {
     p->n = n;
     p->nst = 0;
@@ -87,6 +89,8 @@ int make_fft_plan(int n, FftPlanDev* p)
         p->radix[p->nst++] = r;
         return true;
     };
+    if (radix16)
+        while (m % 16 == 0) { if (!push(16)) return -1; m /= 16; }
     while (m % 8 == 0) { if (!push(8)) return -1; m /= 8; }
     while (m % 4 == 0) { if (!push(4)) return -1; m /= 4; }
     while (m % 2 == 0) { if (!push(2)) return -1; m /= 2; }
@@ -143,9 +147,14 @@ struct fmdbzp_plan {
     int32_t pending_n = 0;
     float* d_export = nullptr;
     int batch = 1;
-    int LB = 1, bstride = 1;
+    int LB = 1, bstride = 1, sst = 1;
+    bool simple_kernel = false;  // column stage runs k_column_simple
+    int tp = 0, ap = 0;          // tile / accumulator row pitches (multiples of 32 elements)
+    bool col_onebuf = false;     // simple column kernel runs its FFT stages in place
+    int corr_variant = 0;        // k_corr_tile: 2 = two in-place transforms per pass, 1 = one, 0 = ping-pong
     size_t col_lds = 0;
     size_t fft_lds = 0;
+    size_t corr_lds = 0;
     float inv_f = 0.f;
     // timing
     bool timing = false;
@@ -256,10 +265,10 @@ int validate_desc(const fmdbzp_plan_desc* d)
     if (d->precision != FMDBZP_PREC_FP32) return set_err(FMDBZP_EINVAL, "unsupported precision %d", d->precision);
     if (d->sampling_freq_hz <= 0) return set_err(FMDBZP_EINVAL, "sampling_freq_hz <= 0");
     // LDS: the F-point transforms need two F-entry buffers in one workgroup
-    if ((size_t)d->fft_size * 2 * sizeof(float2) > kLdsMax)
+    if (((size_t)2 * fft_lds_span(d->fft_size) + d->fft_size) * sizeof(float2) > kLdsMax)
         return set_err(FMDBZP_EINVAL, "fft_size %d exceeds the LDS transform limit", d->fft_size);
     // the column stage needs three ndop-entry complex buffers + two float rows
-    if ((size_t)(d->num_doppler_bins | 1) * 32 > kLdsMax)
+    if ((size_t)(fft_lds_span(d->num_doppler_bins) | 1) * 24 + (size_t)(d->num_doppler_bins | 1) * 12 > kLdsMax)
         return set_err(FMDBZP_EINVAL, "num_doppler_bins %d exceeds the LDS column limit", d->num_doppler_bins);
     int64_t seglen = (int64_t)d->num_coh * d->samples_per_code;
     if (seglen * d->num_noncoherent > ((int64_t)1 << 40)) return set_err(FMDBZP_EINVAL, "signal too long");
@@ -391,35 +400,68 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     };
     if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(FMDBZP_EHIP, "hipStreamCreate failed"));
-    if (make_fft_plan(d.fft_size, &P->pf) || make_fft_plan(d.num_doppler_bins, &P->pd))
+    if (make_fft_plan(d.fft_size, &P->pf, true) || make_fft_plan(d.num_doppler_bins, &P->pd, false))
         return fail(set_err(FMDBZP_EINVAL, "FFT size has too many factors"));
     const int F = d.fft_size, ndop = d.num_doppler_bins, nb = d.num_blocks, bs = d.block_size;
     const int spc = d.samples_per_code, spc_eff = d.samples_per_code_eff;
     P->inv_f = 1.0f / (float)F;
-    P->fft_lds = (size_t)2 * F * sizeof(float2);
+    // Row pitches rounded to 32 elements: every tile row and accumulator row
+    // starts on a 256 / 128-byte boundary, so a workgroup's column segment
+    // does not straddle more cache lines than its size needs.
+    P->tp = (bs + 31) & ~31;
+    P->ap = (spc_eff + 31) & ~31;
+    P->fft_lds = (size_t)2 * fft_lds_span(F) * sizeof(float2);
+    // block correlation: the staged signal-FFT row plus, with in-place
+    // stages, one buffer per concurrent transform (two when they fit the
+    // register bound), else two ping-pong buffers
+    if (fft_inplace_ok(P->pf) && 2 * F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 2;
+    else if (fft_inplace_ok(P->pf) && F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 1;
+    else P->corr_variant = 0;
+    P->corr_lds = (size_t)F * sizeof(float2) +
+                  (size_t)(P->corr_variant == 1 ? 1 : 2) * fft_lds_span(F) * sizeof(float2);
 
     // column-stage geometry: LB columns per workgroup, padded column stride
+    // (odd, so lanes on consecutive columns spread over the LDS banks)
     P->bstride = ndop | 1;
-    // simple: two ping-pong columns; sign/rotation: input column, two
-    // ping-pong columns, stage and raw-power rows
-    const size_t per_col = d.path == FMDBZP_PATH_SIMPLE ? (size_t)P->bstride * 2 * sizeof(float2)
-                                                         : (size_t)P->bstride * 3 * sizeof(float2) +
-                                                               (size_t)ndop * 2 * sizeof(float);
-    // the sign/rotation kernels add ~1.3 KB of static LDS to the dynamic part
-    const size_t budget = d.path == FMDBZP_PATH_SIMPLE ? kLdsDefault : kLdsDefault - 4096;
-    int LB = (int)(budget / per_col);
+    P->sst = ndop | 1;
+    // The sign search with one data bit per coherent block evaluates a single
+    // edge offset (every offset gives the same power spectrum) and a single
+    // all-ones pattern, so it is one column FFT followed by the half-swap fold.
+    // For even ndop the half-swap equals the simple path's (w + ndop/2) mod ndop
+    // fold, and the simple kernel runs it.
+    P->simple_kernel = d.path == FMDBZP_PATH_SIMPLE ||
+                       (d.path == FMDBZP_PATH_SIGN && d.num_data_bits == 1 && ndop % 2 == 0 &&
+                        !(d.options & FMDBZP_OPT_LITERAL_EDGES));
+    // simple: one in-place column buffer; sign/rotation: input column, two
+    // ping-pong columns, stage and raw-power rows (+ one row of pattern signs)
+    // (two ping-pong columns when one column exceeds the in-place register bound)
+    P->col_onebuf = P->simple_kernel && ndop <= FMDBZP_COL_MAXV * FMDBZP_COL_NTHR && fft_inplace_ok(P->pd);
+    const size_t per_col = P->simple_kernel ? (size_t)P->bstride * (P->col_onebuf ? 1 : 2) * sizeof(float2)
+                                            : (size_t)P->bstride * 3 * sizeof(float2) +
+                                                  (size_t)P->sst * 2 * sizeof(float);
+    const size_t fixed = d.path == FMDBZP_PATH_ROTATION ? (size_t)ndop * sizeof(float) : 0;
+    // the sign/rotation kernels add static LDS to the dynamic part
+    const size_t budget = P->simple_kernel ? kLdsDefault : kLdsDefault - 1024;
+    int LB = budget > fixed ? (int)((budget - fixed) / per_col) : 0;
+    if (P->col_onebuf && LB > FMDBZP_COL_MAXV * FMDBZP_COL_NTHR / ndop) LB = FMDBZP_COL_MAXV * FMDBZP_COL_NTHR / ndop;
     if (LB > 64) LB = 64;
-    if (LB >= 16) LB &= ~7;
+    // whole 128-byte accumulator / 256-byte tile segments where possible
+    if (LB >= 32) LB &= ~31;
+    else if (LB >= 16) LB &= ~15;
+    else if (LB >= 8) LB &= ~7;
     if (LB < 1) LB = 1;
     if (LB > bs) LB = bs;
     P->LB = LB;
-    P->col_lds = per_col * LB;
+    P->col_lds = per_col * LB + fixed;
     if (P->col_lds > kLdsMax) return fail(set_err(FMDBZP_EINVAL, "column stage does not fit LDS"));
 
     if ((rc = set_lds_attr((const void*)k_code_fft, P->fft_lds)) ||
         (rc = set_lds_attr((const void*)k_signal_fft, P->fft_lds)) ||
-        (rc = set_lds_attr((const void*)k_corr_tile, P->fft_lds)) ||
-        (rc = set_lds_attr((const void*)k_column_simple, P->col_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<2, true>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<1, true>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<1, false>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_simple<true>, P->col_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_simple<false>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_sign, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_rotation, P->col_lds)))
         return fail(rc);
@@ -482,7 +524,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // workspace: signal FFT cache for all segments + PRN-batch tile/accumulator
     if ((rc = dev_alloc(&P->d_sigfft, (size_t)d.num_noncoherent * ndop * F))) return fail(rc);
     {
-        const size_t per_prn = (size_t)nb * ndop * bs * sizeof(float2) + (size_t)ndop * spc_eff * sizeof(float);
+        const size_t per_prn = (size_t)nb * ndop * P->tp * sizeof(float2) + (size_t)ndop * P->ap * sizeof(float);
         size_t budget = workspace_budget();
         size_t b = budget / per_prn;
         if (b < 1) b = 1;
@@ -490,8 +532,8 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         if (b > 65535) b = 65535;
         P->batch = (int)b;
     }
-    if ((rc = dev_alloc(&P->d_tile, (size_t)P->batch * nb * ndop * bs)) ||
-        (rc = dev_alloc(&P->d_acc, (size_t)P->batch * ndop * spc_eff)) ||
+    if ((rc = dev_alloc(&P->d_tile, (size_t)P->batch * nb * ndop * P->tp)) ||
+        (rc = dev_alloc(&P->d_acc, (size_t)P->batch * ndop * P->ap)) ||
         (rc = dev_alloc(&P->d_rowsum, (size_t)P->batch * ndop)) ||
         (rc = dev_alloc(&P->d_rowbest_v, (size_t)P->batch * ndop)) ||
         (rc = dev_alloc(&P->d_rowbest_k, (size_t)P->batch * ndop)))
@@ -616,6 +658,10 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
     A.use_max = (ndc > 1 || d.bit_edge_search_steps > 1) ? 1 : 0;
     A.LB = P->LB;
     A.bstride = P->bstride;
+    A.sst = P->sst;
+    A.literal_edges = (d.options & FMDBZP_OPT_LITERAL_EDGES) ? 1 : 0;
+    A.tp = P->tp;
+    A.ap = P->ap;
     const int streamed_rot = (nnc == 1 && d.path == FMDBZP_PATH_ROTATION) ? 1 : 0;
     const int nlb = (bs + P->LB - 1) / P->LB;
 
@@ -628,9 +674,17 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
         for (int s = 0; s < nnc; s++) {
             {
                 Timed t(P, FMDBZP_K_CORR_TILE);
-                k_corr_tile<<<dim3(ndop, nb, nbat), FMDBZP_NTHR, P->fft_lds, P->stream>>>(
-                    P->d_sigfft + (size_t)s * ndop * F, P->d_prnconj, req, P->d_tile, P->pf, P->d_twF, nb, bs, ndop,
-                    P->inv_f);
+                const dim3 g(ndop, nbat);
+                const float2* sf = P->d_sigfft + (size_t)s * ndop * F;
+                if (P->corr_variant == 2)
+                    k_corr_tile<2, true><<<g, FMDBZP_NTHR, P->corr_lds, P->stream>>>(
+                        sf, P->d_prnconj, req, P->d_tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
+                else if (P->corr_variant == 1)
+                    k_corr_tile<1, true><<<g, FMDBZP_NTHR, P->corr_lds, P->stream>>>(
+                        sf, P->d_prnconj, req, P->d_tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
+                else
+                    k_corr_tile<1, false><<<g, FMDBZP_NTHR, P->corr_lds, P->stream>>>(
+                        sf, P->d_prnconj, req, P->d_tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
                 HIP_TRY(hipGetLastError());
             }
             A.shifts = P->d_shifts + (size_t)s * ndop;
@@ -638,8 +692,13 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
             {
                 Timed t(P, FMDBZP_K_COLUMN);
                 dim3 g(nlb, nb, nbat);
-                if (d.path == FMDBZP_PATH_SIMPLE)
-                    k_column_simple<<<g, FMDBZP_NTHR, P->col_lds, P->stream>>>(A);
+                if (P->simple_kernel)
+                {
+                    if (P->col_onebuf)
+                        k_column_simple<true><<<g, FMDBZP_COL_NTHR, P->col_lds, P->stream>>>(A);
+                    else
+                        k_column_simple<false><<<g, FMDBZP_COL_NTHR, P->col_lds, P->stream>>>(A);
+                }
                 else if (d.path == FMDBZP_PATH_SIGN)
                     k_column_sign<<<g, FMDBZP_NTHR, P->col_lds, P->stream>>>(A);
                 else
@@ -650,14 +709,14 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
         {
             Timed t(P, FMDBZP_K_REDUCE);
             k_reduce_rows<<<dim3(ndop, nbat), FMDBZP_NTHR, 0, P->stream>>>(
-                P->d_acc, ndop, spc, spc_eff, d.num_secondary_rotations, streamed_rot, P->d_rowsum, P->d_rowbest_v,
+                P->d_acc, ndop, spc, spc_eff, P->ap, d.num_secondary_rotations, streamed_rot, P->d_rowsum, P->d_rowbest_v,
                 P->d_rowbest_k);
             HIP_TRY(hipGetLastError());
         }
         {
             Timed t(P, FMDBZP_K_FINALIZE);
             k_finalize<<<dim3(nbat), FMDBZP_NTHR, 0, P->stream>>>(
-                P->d_acc, P->d_rowsum, P->d_rowbest_v, P->d_rowbest_k, ndop, spc, spc_eff, d.num_secondary_rotations,
+                P->d_acc, P->d_rowsum, P->d_rowbest_v, P->d_rowbest_k, ndop, spc, spc_eff, P->ap, d.num_secondary_rotations,
                 streamed_rot, d.noise_estimator, d.samples_per_chip, P->d_stats + b0);
             HIP_TRY(hipGetLastError());
         }
@@ -670,8 +729,8 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                 {
                     Timed t(P, FMDBZP_K_EXPORT);
                     dim3 g((spc_eff + 31) / 32, (ndop + 31) / 32);
-                    k_export<<<g, dim3(32, 8), 0, P->stream>>>(P->d_acc + (size_t)p * ndop * spc_eff, target, ndop,
-                                                                spc_eff);
+                    k_export<<<g, dim3(32, 8), 0, P->stream>>>(P->d_acc + (size_t)p * ndop * P->ap, target, ndop,
+                                                                spc_eff, P->ap);
                     HIP_TRY(hipGetLastError());
                 }
                 if (!dev_out)

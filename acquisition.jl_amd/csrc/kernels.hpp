@@ -15,6 +15,30 @@
 #include "../../include/fmdbzp.h"
 
 #define FMDBZP_NTHR 256
+// The simple column kernel runs 512 threads, each holding at most
+// FMDBZP_COL_MAXV complex values per in-place FFT stage:
+// LB * ndop <= FMDBZP_COL_MAXV * FMDBZP_COL_NTHR.
+#define FMDBZP_COL_NTHR 512
+#define FMDBZP_COL_MAXV 16
+// complex values a thread holds per in-place stage of the block-correlation
+// transforms: NT * F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR
+#define FMDBZP_CORR_MAXV 16
+
+// ---------------------------------------------------------------------------
+// XCD-aware block order. The dispatcher hands workgroup b (linear id) to XCD
+// b mod 8; each XCD has its own L2. Remapping gives every XCD one contiguous
+// range of logical blocks, so neighbouring tiles (which share cache lines of
+// the accumulator or the same code FFTs) meet in the same L2.
+// ---------------------------------------------------------------------------
+#define FMDBZP_NUM_XCD 8
+__device__ __forceinline__ int xcd_logical_block()
+{
+    const int nblk = gridDim.x * gridDim.y * gridDim.z;
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int per = nblk / FMDBZP_NUM_XCD, rem = nblk % FMDBZP_NUM_XCD;
+    const int x = bid % FMDBZP_NUM_XCD, s = bid / FMDBZP_NUM_XCD;
+    return x * per + min(x, rem) + s;
+}
 
 // ComplexF32(x) of one input sample (src/acquire.jl:94, :101).
 __device__ __forceinline__ float2 load_sample(const void* __restrict__ sig, int eltype, int64_t i)
@@ -81,7 +105,9 @@ k_code_fft(const int8_t* __restrict__ codes, float2* __restrict__ prnconj, FftPl
     const int blk = blockIdx.x, prn = blockIdx.y;
     const int8_t* code = codes + (size_t)prn * spc + (size_t)blk * bs;
     float2* out = prnconj + ((size_t)prn * nb + blk) * F;
-    fft_run<false, false>(lds, lds + F, pf, twF, 1, F, threadIdx.x, FMDBZP_NTHR, CodeIn{code, bs}, ConjOut{out});
+    const int span = fft_lds_span(F);
+    fft_run<false, false>(lds, lds + span, pf, twF, 1, span, threadIdx.x, FMDBZP_NTHR, CodeIn{code, bs},
+                          ConjOut{out});
 }
 
 // ---------------------------------------------------------------------------
@@ -134,7 +160,8 @@ k_signal_fft(const void* __restrict__ sig, int eltype, float2* __restrict__ sigf
     DownconvIn in{sig, eltype, seg0, seglen, (int64_t)gk * bs, 2 * bs, do_if,
                   __fmul_rn((float)seg0, phase_step) /* Float32(seg_start-1) * step */, phase_step};
     float2* out = sigfft + ((size_t)step * ndop + gk) * F;
-    fft_run<false, false>(lds, lds + F, pf, twF, 1, F, threadIdx.x, FMDBZP_NTHR, in, PlainOut{out});
+    const int span = fft_lds_span(F);
+    fft_run<false, false>(lds, lds + span, pf, twF, 1, span, threadIdx.x, FMDBZP_NTHR, in, PlainOut{out});
 }
 
 // ---------------------------------------------------------------------------
@@ -143,36 +170,66 @@ k_signal_fft(const void* __restrict__ sig, int eltype, float2* __restrict__ sigf
 // (k mod nb + r) mod nb: pointwise product in the frequency domain (formed in
 // registers by the first FFT stage), inverse FFT (unnormalised), times
 // Float32(1/F); the last stage stores only the first bs lags.
-// grid (ndop, nb, batch)
 // ---------------------------------------------------------------------------
 struct CorrIn {
-    const float2* S;
-    const float2* P;
-    __device__ __forceinline__ float2 operator()(int, int i) const { return cmulf(S[i], P[i]); }
+    const float2* S;   // signal-FFT row in LDS
+    const float2* P;   // conj code sub-block FFTs of this PRN, [nb][F]
+    int F, nb, b0;     // transform t uses code block (b0 + t) mod nb
+    __device__ __forceinline__ float2 operator()(int t, int i) const
+    {
+        int b = b0 + t;
+        if (b >= nb) b -= nb;
+        return cmulf(S[i], P[(size_t)b * F + i]);
+    }
 };
 struct LagOut {
-    float2* out;
+    float2* out;        // tile row of transform 0
+    size_t tstride;     // tile distance between consecutive column blocks
     int bs;
     float inv_f;
-    __device__ __forceinline__ void operator()(int, int i, float2 v) const
+    __device__ __forceinline__ void operator()(int t, int i, float2 v) const
     {
-        if (i < bs) out[i] = make_float2(v.x * inv_f, v.y * inv_f);
+        if (i < bs) out[t * tstride + i] = make_float2(v.x * inv_f, v.y * inv_f);
     }
 };
 
+// One workgroup per (row k, PRN): the signal-FFT row is staged in LDS once and
+// reused for all nb column blocks r, NT blocks per pass. ONEBUF: in-place
+// stages, one LDS buffer per concurrent transform; else two ping-pong buffers
+// (NT = 1).
+// grid (ndop, batch)
+template <int NT, bool ONEBUF>
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
             const int32_t* __restrict__ req, float2* __restrict__ tile, FftPlanDev pf,
-            const float2* __restrict__ twF, int nb, int bs, int ndop, float inv_f)
+            const float2* __restrict__ twF, int nb, int bs, int tp, int ndop, float inv_f)
 {
     extern __shared__ float2 lds[];
     const int F = pf.n;
-    const int k = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int span = fft_lds_span(F);
+    float2* srow = lds;
+    float2* a = lds + F;
+    const int lbk = xcd_logical_block();
+    const int p = lbk / gridDim.x, k = lbk - p * gridDim.x;
+    const int tid = threadIdx.x;
     const int prn = req[p];
-    const int pb = (k % nb + r) % nb;
-    CorrIn in{sigfft_step + (size_t)k * F, prnconj + ((size_t)prn * nb + pb) * F};
-    LagOut out{tile + (((size_t)p * nb + r) * ndop + k) * bs, bs, inv_f};
-    fft_run<true, false>(lds, lds + F, pf, twF, 1, F, threadIdx.x, FMDBZP_NTHR, in, out);
+    const float2* S = sigfft_step + (size_t)k * F;
+    for (int i = tid; i < F; i += FMDBZP_NTHR) srow[i] = S[i];
+    const float2* Pp = prnconj + (size_t)prn * nb * F;
+    const int kb = k % nb;
+#pragma unroll 1
+    for (int r0 = 0; r0 < nb; r0 += NT) {
+        __syncthreads();  // srow ready / previous transforms done with the buffers
+        const int nt = min(NT, nb - r0);
+        int b0 = kb + r0;
+        if (b0 >= nb) b0 -= nb;
+        CorrIn in{srow, Pp, F, nb, b0};
+        LagOut out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (size_t)ndop * tp, bs, inv_f};
+        if constexpr (ONEBUF)
+            fft_run_1buf<true, false, FMDBZP_CORR_MAXV>(a, pf, twF, nt, span, tid, FMDBZP_NTHR, in, out);
+        else
+            fft_run<true, false>(a, a + span, pf, twF, 1, span, tid, FMDBZP_NTHR, in, out);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -181,8 +238,8 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
 // loads and accumulator read-modify-writes are coalesced along code columns.
 // ---------------------------------------------------------------------------
 struct ColumnArgs {
-    const float2* tile;      // [batch][nb][ndop][bs]
-    float* acc;              // [batch][ndop][spc_eff]
+    const float2* tile;      // [batch][nb][ndop][tp], lags < bs used
+    float* acc;              // [batch][ndop][ap], columns < spc_eff used
     const int32_t* shifts;   // [ndop] drift shifts of this step, sorted-row keyed, in [0, spc)
     const float* patterns;   // [nsets][Q][ncoh] (column-major (ncoh, Q) per set)
     const int32_t* pset;     // [nprn] pattern set per PRN
@@ -190,7 +247,8 @@ struct ColumnArgs {
     const float2* twD;       // column FFT twiddles
     FftPlanDev pd;
     int nb, bs, ndop, spc, spc_eff, ncoh, ndb, E, R, Q, ndc, edge_step, use_max;
-    int LB, bstride, first_step;
+    int LB, bstride, sst, first_step, literal_edges;
+    int tp, ap;  // tile and accumulator row pitches (elements, multiples of 32)
 };
 
 // Accumulate (or store, at the first step) one power value with the drift
@@ -200,7 +258,7 @@ __device__ __forceinline__ void acc_cell(const ColumnArgs& A, int p, int row, in
 {
     int dc = cp + A.shifts[row];
     if (dc >= A.spc) dc -= A.spc;
-    float* dst = A.acc + ((size_t)p * A.ndop + row) * A.spc_eff + (size_t)slice * A.spc + dc;
+    float* dst = A.acc + ((size_t)p * A.ndop + row) * A.ap + (size_t)slice * A.spc + dc;
     if (A.first_step) *dst = v;
     else *dst = *dst + v;
 }
@@ -226,28 +284,34 @@ struct SimpleAccOut {
 
 // `_accumulate_fftshifted_power_(drift_)pilot!` (src/noncoherent_integration.jl:482-592)
 // and `_stream_power_column!` (src/acquire.jl:335-385).
-__global__ void __launch_bounds__(FMDBZP_NTHR)
+template <bool ONEBUF>
+__global__ void __launch_bounds__(FMDBZP_COL_NTHR, 4)
 k_column_simple(ColumnArgs A)
 {
     extern __shared__ float2 lds[];
-    const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int lbk = xcd_logical_block();
+    const int lb = lbk % gridDim.x, r = (lbk / gridDim.x) % gridDim.y, p = lbk / (gridDim.x * gridDim.y);
     const int lag0 = lb * A.LB;
     const int ncol = min(A.LB, A.bs - lag0);
     const int bst = A.bstride;
-    const float2* T = A.tile + (((size_t)p * A.nb + r) * A.ndop) * A.bs + lag0;
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * A.ndop) * A.tp + lag0;
     SimpleAccOut out{&A, p, r * A.bs + lag0, A.ndop / 2};
-    fft_run<false, true>(lds, lds + (size_t)A.LB * bst, A.pd, A.twD, ncol, bst, threadIdx.x, FMDBZP_NTHR,
-                         TileColIn{T, A.bs}, out);
+    if constexpr (ONEBUF)
+        fft_run_1buf<false, true, FMDBZP_COL_MAXV>(lds, A.pd, A.twD, ncol, bst, threadIdx.x, FMDBZP_COL_NTHR,
+                                                    TileColIn{T, A.tp}, out);
+    else
+        fft_run<false, true>(lds, lds + (size_t)A.LB * bst, A.pd, A.twD, ncol, bst, threadIdx.x, FMDBZP_COL_NTHR,
+                             TileColIn{T, A.tp}, out);
 }
 
-// Power of each raw bin into an LDS float array (row-major per column).
+// Power of each raw bin into an LDS float array (stride sst per column).
 template <bool ROT>
 struct PowerOut {
     float* pw;
-    int ndop;
+    int sst;
     __device__ __forceinline__ void operator()(int t, int w, float2 z) const
     {
-        pw[t * ndop + w] = ROT ? fmaf(z.x, z.x, __fmul_rn(z.y, z.y)) : abs2_exact(z);
+        pw[t * sst + w] = ROT ? fmaf(z.x, z.x, __fmul_rn(z.y, z.y)) : abs2_exact(z);
     }
 };
 
@@ -256,24 +320,26 @@ struct PowerOut {
 // goes to w + half, raw w >= half to w - half, first loop then second. For odd
 // ndop both raw 0 and raw ndop-1 land on row `half` and row ndop-1 is never
 // written; iterating destinations reproduces that exactly.
-__device__ __forceinline__ void fold_into_stage(float* stage, const float* pw, int ncol, int ndop, int use_max,
-                                                int tid)
+__device__ __forceinline__ void fold_into_stage(float* stage, const float* pw, int ncol, int ndop, int sst,
+                                                int use_max, int tid)
 {
     const int half = ndop / 2;
+    const float inv = 1.0f / (float)ndop;
     for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-        const int c = idx / ndop, dst = idx - c * ndop;
-        float cur = stage[idx];
+        int dst;
+        const int c = fdivmod(idx, ndop, inv, &dst);
+        float cur = stage[c * sst + dst];
         if (dst >= half && dst < 2 * half) {  // first loop: raw dst - half
-            const float v = pw[c * ndop + dst - half];
+            const float v = pw[c * sst + dst - half];
             if (!use_max) cur = v;
             else if (v > cur) cur = v;
         }
         if (dst < ndop - half) {  // second loop: raw dst + half
-            const float v = pw[c * ndop + dst + half];
+            const float v = pw[c * sst + dst + half];
             if (!use_max) cur = v;
             else if (v > cur) cur = v;
         }
-        stage[idx] = cur;
+        stage[c * sst + dst] = cur;
     }
 }
 
@@ -282,9 +348,11 @@ __device__ __forceinline__ void fold_into_stage(float* stage, const float* pw, i
 __device__ __forceinline__ void scatter_stage(const ColumnArgs& A, const float* stage, int p, int r,
                                               int lag0, int ncol, int slice, int tid)
 {
+    const float inv = 1.0f / (float)ncol;
     for (int idx = tid; idx < ncol * A.ndop; idx += FMDBZP_NTHR) {
-        const int row = idx / ncol, c = idx - row * ncol;
-        acc_cell(A, p, row, r * A.bs + lag0 + c, slice, stage[c * A.ndop + row]);
+        int c;
+        const int row = fdivmod(idx, ncol, inv, &c);
+        acc_cell(A, p, row, r * A.bs + lag0 + c, slice, stage[c * A.sst + row]);
     }
 }
 
@@ -292,8 +360,10 @@ __device__ __forceinline__ void scatter_stage(const ColumnArgs& A, const float* 
 __device__ __forceinline__ void load_columns(float2* x, const float2* T, int bs, int ndop, int ncol, int bst,
                                              int tid)
 {
+    const float inv = 1.0f / (float)ncol;
     for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) {
-        const int k = idx / ncol, c = idx - k * ncol;
+        int c;
+        const int k = fdivmod(idx, ncol, inv, &c);
         x[c * bst + k] = T[(size_t)k * bs + c];
     }
 }
@@ -309,7 +379,8 @@ struct SignFoldIn {
         float2 v = make_float2(0.f, 0.f);
         if (n < sub_len) {
             for (int b = 0; b < ndb; b++) {
-                int src = (ro + b * sub_len + n) % ndop;
+                int src = ro + b * sub_len + n;  // < 2 ndop
+                if (src >= ndop) src -= ndop;
                 const float2 u = x[t * bst + src];
                 v.x += sg[b] * u.x;
                 v.y += sg[b] * u.y;
@@ -325,35 +396,40 @@ struct SignFoldIn {
 // the pattern-signed FFTs of the N_db zero-padded sub-blocks; by linearity
 // that is the FFT of the signed fold of the sub-blocks onto [0, ndop/N_db),
 // zero-padded, which is what is transformed here (one FFT per pattern).
+// With N_db == 1 the edge offsets only rotate the column circularly, which
+// multiplies its FFT by a unit phase ramp (shift theorem), so every edge gives
+// the same |FFT|^2 and one edge is evaluated unless literal_edges is set.
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_column_sign(ColumnArgs A)
 {
     extern __shared__ float2 lds[];
     __shared__ float sgn[32];
     const int tid = threadIdx.x;
-    const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int lbk = xcd_logical_block();
+    const int lb = lbk % gridDim.x, r = (lbk / gridDim.x) % gridDim.y, p = lbk / (gridDim.x * gridDim.y);
     const int lag0 = lb * A.LB;
     const int ncol = min(A.LB, A.bs - lag0);
-    const int ndop = A.ndop, bst = A.bstride;
+    const int ndop = A.ndop, bst = A.bstride, sst = A.sst;
     float2* x = lds;
     float2* ya = x + (size_t)A.LB * bst;
     float2* yb = ya + (size_t)A.LB * bst;
     float* stage = (float*)(yb + (size_t)A.LB * bst);
-    float* pw = stage + (size_t)A.LB * ndop;
-    load_columns(x, A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0, A.bs, ndop, ncol, bst, tid);
-    for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) stage[idx] = 0.f;
+    float* pw = stage + (size_t)A.LB * sst;
+    load_columns(x, A.tile + (((size_t)p * A.nb + r) * ndop) * A.tp + lag0, A.tp, ndop, ncol, bst, tid);
+    for (int idx = tid; idx < A.LB * sst; idx += FMDBZP_NTHR) stage[idx] = 0.f;
     const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
     const int sub_len = ndop / A.ndb;
     const int cpb = A.ncoh / A.ndb;
-    for (int e = 0; e < A.E; e++) {
+    const int E = (A.ndb == 1 && !A.literal_edges) ? 1 : A.E;
+    for (int e = 0; e < E; e++) {
         for (int d = 0; d < A.Q; d++) {
             __syncthreads();
             if (tid < A.ndb) sgn[tid] = pat[(size_t)d * A.ncoh + tid * cpb];
             __syncthreads();
             SignFoldIn in{x, sgn, bst, ndop, A.ndb, sub_len, e * A.edge_step};
-            fft_run<false, true>(ya, yb, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR, in, PowerOut<false>{pw, ndop});
+            fft_run<false, true>(ya, yb, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR, in, PowerOut<false>{pw, sst});
             __syncthreads();
-            fold_into_stage(stage, pw, ncol, ndop, 1, tid);
+            fold_into_stage(stage, pw, ncol, ndop, sst, 1, tid);
         }
     }
     __syncthreads();
@@ -361,17 +437,17 @@ k_column_sign(ColumnArgs A)
 }
 
 // Rotation search input: the column rotated by the per-column offset, times
-// the +-1 pattern of coherent period n / nb.
+// the +-1 pattern of coherent period n / nb (expanded per row in LDS).
 struct RotIn {
     const float2* x;
-    const float* pq;   // pattern q, one sign per coherent period
-    const int* eff;    // per-column row offset
-    int bst, ndop, nb;
+    const float* sgn_row;  // pattern sign of each row n
+    const int* eff;        // per-column row offset
+    int bst, ndop;
     __device__ __forceinline__ float2 operator()(int t, int n) const
     {
         int src = eff[t] + n;
         if (src >= ndop) src -= ndop;
-        const float sg = pq[n / nb];
+        const float sg = sgn_row[n];
         const float2 u = x[t * bst + src];
         return make_float2(sg * u.x, sg * u.y);
     }
@@ -390,23 +466,25 @@ __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_column_rotation(ColumnArgs A)
 {
     extern __shared__ float2 lds[];
-    __shared__ float spat[256];
     __shared__ int seff[64];
     const int tid = threadIdx.x;
-    const int lb = blockIdx.x, r = blockIdx.y, p = blockIdx.z;
+    const int lbk = xcd_logical_block();
+    const int lb = lbk % gridDim.x, r = (lbk / gridDim.x) % gridDim.y, p = lbk / (gridDim.x * gridDim.y);
     const int lag0 = lb * A.LB;
     const int ncol = min(A.LB, A.bs - lag0);
-    const int ndop = A.ndop, bst = A.bstride;
+    const int ndop = A.ndop, bst = A.bstride, sst = A.sst;
     float2* x = lds;
     float2* ya = x + (size_t)A.LB * bst;
     float2* yb = ya + (size_t)A.LB * bst;
     float* stage = (float*)(yb + (size_t)A.LB * bst);
-    float* pw = stage + (size_t)A.LB * ndop;
-    load_columns(x, A.tile + (((size_t)p * A.nb + r) * ndop) * A.bs + lag0, A.bs, ndop, ncol, bst, tid);
+    float* pw = stage + (size_t)A.LB * sst;
+    float* sgn_row = pw + (size_t)A.LB * sst;
+    load_columns(x, A.tile + (((size_t)p * A.nb + r) * ndop) * A.tp + lag0, A.tp, ndop, ncol, bst, tid);
     const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
     const int half_block = A.bs >> 1;
+    const float inv_nb = 1.0f / (float)A.nb;
     for (int rot = 0; rot < A.R; rot++) {
-        for (int idx = tid; idx < ncol * ndop; idx += FMDBZP_NTHR) stage[idx] = 0.f;
+        for (int idx = tid; idx < A.LB * sst; idx += FMDBZP_NTHR) stage[idx] = 0.f;
         for (int e = 0; e < A.E; e++) {
             __syncthreads();
             // per-column alignment to the primary-code wrap (:212-215)
@@ -419,12 +497,15 @@ k_column_rotation(ColumnArgs A)
             for (int d = 0; d < A.ndc; d++) {
                 const int q = rot * A.ndc + d;
                 __syncthreads();
-                for (int i = tid; i < A.ncoh; i += FMDBZP_NTHR) spat[i] = pat[(size_t)q * A.ncoh + i];
+                for (int n = tid; n < ndop; n += FMDBZP_NTHR) {
+                    int rem;
+                    sgn_row[n] = pat[(size_t)q * A.ncoh + fdivmod(n, A.nb, inv_nb, &rem)];
+                }
                 __syncthreads();
-                RotIn in{x, spat, seff, bst, ndop, A.nb};
-                fft_run<false, true>(ya, yb, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR, in, PowerOut<true>{pw, ndop});
+                RotIn in{x, sgn_row, seff, bst, ndop};
+                fft_run<false, true>(ya, yb, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR, in, PowerOut<true>{pw, sst});
                 __syncthreads();
-                fold_into_stage(stage, pw, ncol, ndop, A.use_max, tid);
+                fold_into_stage(stage, pw, ncol, ndop, sst, A.use_max, tid);
             }
         }
         __syncthreads();
@@ -457,14 +538,14 @@ __device__ __forceinline__ bool better(float v, uint64_t k, float bv, uint64_t b
 
 // grid (ndop, batch): one workgroup per (row, PRN).
 __global__ void __launch_bounds__(FMDBZP_NTHR)
-k_reduce_rows(const float* __restrict__ acc, int ndop, int spc, int spc_eff, int R, int streamed_rot,
+k_reduce_rows(const float* __restrict__ acc, int ndop, int spc, int spc_eff, int ap, int R, int streamed_rot,
               double* __restrict__ rowsum, float* __restrict__ rowbest_v, uint64_t* __restrict__ rowbest_k)
 {
     __shared__ double ssum[FMDBZP_NTHR];
     __shared__ float sv[FMDBZP_NTHR];
     __shared__ uint64_t sk[FMDBZP_NTHR];
     const int row = blockIdx.x, p = blockIdx.y, tid = threadIdx.x;
-    const float* a = acc + ((size_t)p * ndop + row) * spc_eff;
+    const float* a = acc + ((size_t)p * ndop + row) * ap;
     double s = 0.0;
     float bv = -INFINITY;
     uint64_t bk = ~0ull;
@@ -500,7 +581,7 @@ k_reduce_rows(const float* __restrict__ acc, int ndop, int spc, int spc_eff, int
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
            const float* __restrict__ rowbest_v, const uint64_t* __restrict__ rowbest_k, int ndop,
-           int spc, int spc_eff, int R, int streamed_rot, int noise_est, int spchip,
+           int spc, int spc_eff, int ap, int R, int streamed_rot, int noise_est, int spchip,
            fmdbzp_prn_stats* __restrict__ stats)
 {
     __shared__ double ssum[FMDBZP_NTHR];
@@ -556,7 +637,7 @@ k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
     }
     __syncthreads();
     const int row = s_row, col = s_col;
-    const float* a = acc + (size_t)p * ndop * spc_eff;
+    const float* a = acc + (size_t)p * ndop * ap;
     // GlobalMean exclusion: columns [col - spchip, col + spchip] clipped, all rows
     int lo = col - spchip;
     if (lo < 0) lo = 0;
@@ -567,7 +648,7 @@ k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
     if (noise_est == FMDBZP_NOISE_GLOBAL_MEAN) {
         for (int idx = tid; idx < nex * ndop; idx += FMDBZP_NTHR) {
             const int rr = idx / nex, cc = lo + idx - rr * nex;
-            ex += (double)a[(size_t)rr * spc_eff + cc];
+            ex += (double)a[(size_t)rr * ap + cc];
         }
     }
     __syncthreads();
@@ -600,24 +681,24 @@ k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
         const int cr = (scr == spc - 1 ? 0 : scr + 1) + rot * spc;
         const int dl = row == 0 ? ndop - 1 : row - 1;
         const int dr = row == ndop - 1 ? 0 : row + 1;
-        st.code_left = a[(size_t)row * spc_eff + cl];
-        st.code_right = a[(size_t)row * spc_eff + cr];
-        st.dop_left = a[(size_t)dl * spc_eff + col];
-        st.dop_right = a[(size_t)dr * spc_eff + col];
+        st.code_left = a[(size_t)row * ap + cl];
+        st.code_right = a[(size_t)row * ap + cr];
+        st.dop_left = a[(size_t)dl * ap + col];
+        st.dop_right = a[(size_t)dr * ap + col];
         stats[p] = st;
     }
 }
 
-// Surface export for store_power_bins: acc[row][col] -> Julia column-major
+// Surface export for store_power_bins: acc[row][col] (row pitch ap) -> Julia column-major
 // (ndop, spc_eff), i.e. out[col * ndop + row]. 32x32 LDS-tiled transpose.
 // grid (ceil(spc_eff/32), ceil(ndop/32)), block (32, 8)
-__global__ void k_export(const float* __restrict__ acc, float* __restrict__ out, int ndop, int spc_eff)
+__global__ void k_export(const float* __restrict__ acc, float* __restrict__ out, int ndop, int spc_eff, int ap)
 {
     __shared__ float t[32][33];
     const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
     for (int i = threadIdx.y; i < 32; i += 8) {
         const int r = r0 + i, c = c0 + threadIdx.x;
-        if (r < ndop && c < spc_eff) t[i][threadIdx.x] = acc[(size_t)r * spc_eff + c];
+        if (r < ndop && c < spc_eff) t[i][threadIdx.x] = acc[(size_t)r * ap + c];
     }
     __syncthreads();
     for (int i = threadIdx.y; i < 32; i += 8) {

@@ -30,16 +30,96 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_calibration(nbytes: int):
+    """Time the plain copy kernel of libhbmcopy.so. Profiled with the
+    FETCH_SIZE / WRITE_SIZE counters, its per-dispatch readings over
+    `nbytes` are the scale factors for the engine kernels' readings."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "acquisition.jl_amd", "acquisition", "_lib", "libhbmcopy.so"))
+    lib.hbm_copy_run.argtypes = [ctypes.c_size_t, ctypes.c_int]
+    lib.hbm_copy_run.restype = ctypes.c_double
+    ms = lib.hbm_copy_run(nbytes, 5)
+    if ms <= 0:
+        raise SystemExit(f"copy calibration failed ({ms})")
+    print(json.dumps({"copy_bytes": nbytes, "read_kib": nbytes / 1024, "write_kib": nbytes / 1024,
+                      "ms_per_copy": round(ms, 4), "gb_per_s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}),
+          flush=True)
+
+
+KERNEL_CLASS_OF = {"k_corr_tile": "corr_tile", "k_column_simple": "column", "k_column_sign": "column",
+                   "k_column_rotation": "column", "k_signal_fft": "signal_fft", "k_reduce_rows": "reduce",
+                   "k_finalize": "finalize", "k_code_fft": "code_fft"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_l1ca_16mhz_32prn.json")
+
+
+def pmc_summary(base: str, config_key: str):
+    """Per-dispatch HBM bytes of each engine kernel class from four rocprofv3
+    --pmc CSV outputs under `base` (pmc_fetch, pmc_write: this workload;
+    pmc_copy_fetch, pmc_copy_write: the plain copy of --copy-calibration),
+    each counter scaled by copied bytes / counter reading of the copy kernel."""
+    import csv
+
+    def per_dispatch(name, counter):
+        sums, counts = {}, {}
+        with open(os.path.join(base, name, "run_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                k = r["Kernel_Name"].split("(")[0].split("<")[0].replace("void ", "").strip()
+                sums[k] = sums.get(k, 0.0) + float(r["Counter_Value"])
+                counts[k] = counts.get(k, 0) + 1
+        return {k: sums[k] / counts[k] for k in sums}
+
+    cf, cw = per_dispatch("pmc_copy_fetch", "FETCH_SIZE"), per_dispatch("pmc_copy_write", "WRITE_SIZE")
+    copy_kib = 1024.0 * 1024.0  # --copy-calibration 1024 (MiB)
+    fs, ws = copy_kib / cf["hbm_copy_calibration"], copy_kib / cw["hbm_copy_calibration"]
+    f, w = per_dispatch("pmc_fetch", "FETCH_SIZE"), per_dispatch("pmc_write", "WRITE_SIZE")
+    out = {"config_key": config_key, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4), "kernels": {}}
+    for k in f:
+        cls = KERNEL_CLASS_OF.get(k)
+        if cls and k in w:
+            out["kernels"][cls] = {"kernel": k, "fetch_kib": round(f[k], 1), "write_kib": round(w[k], 1),
+                                   "hbm_bytes": int(f[k] * fs * 1024 + w[k] * ws * 1024)}
+    print(json.dumps(out, indent=1))
+
+
+def measured_traffic(kernel_class: str, config_key: str):
+    """HBM bytes per launch of `kernel_class` recorded by pmc_summary for this
+    workload configuration, or None when no matching record exists."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("config_key") != config_key:
+        return None
+    k = rec.get("kernels", {}).get(kernel_class)
+    return k["hbm_bytes"] if k else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--edge-steps", type=int, default=10)
+    ap.add_argument("--literal-edges", action="store_true",
+                    help="evaluate every bit-edge offset (the default evaluates one: with one data bit per "
+                         "coherent block the offsets are circular rotations of the column)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-prns", type=int, default=16)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (FETCH_SIZE/WRITE_SIZE counter calibration)")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="only summarise rocprofv3 FETCH_SIZE/WRITE_SIZE CSV runs under DIR")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, f"E{args.edge_steps}-{'literal' if args.literal_edges else 'one-edge'}")
+        return
+    if args.copy_calibration > 0:
+        copy_calibration(int(args.copy_calibration * 2**20))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the engine runs on a single MI355X; see the sharding notes in DESIGN.md)", file=sys.stderr)
@@ -56,7 +136,7 @@ def main():
     prns = list(range(1, 33))
     plan = plan_acquire(sc.system, sc.sampling_freq, prns, num_coherently_integrated_code_periods=10,
                         bit_edge_search_steps=args.edge_steps, num_noncoherent_accumulations=8,
-                        min_doppler_coverage=7000.0)
+                        min_doppler_coverage=7000.0, literal_edges=args.literal_edges)
     dev = DeviceArray(sc.signal)
     ndop, spc, R = plan.num_doppler_bins, plan.samples_per_code, plan.num_secondary_rotations
     cells = len(prns) * ndop * spc * R
@@ -79,6 +159,13 @@ def main():
     log(f"timed {args.steps} steps: {dt * 1e3:.3f} ms/step; kernel ms totals "
         + ", ".join(f"{k} {v[0]:.2f}/{v[1]}" for k, v in kt.items() if v[1]))
 
+    # the same steps with the samples in host memory (pageable numpy array:
+    # the library stages them through pinned memory and copies them over PCIe)
+    t0 = time.perf_counter()
+    for i in range(max(1, args.steps // 2)):
+        acquire_(plan, sc.signal, prns, interm_freq=sc.interm_freq)
+    dt_host = (time.perf_counter() - t0) / max(1, args.steps // 2)
+
     # roofline of the dominant kernel (algorithmic bytes per launch)
     F, nb = plan.fft_size, plan.num_blocks
     nnc = plan.num_noncoherent_accumulations
@@ -98,8 +185,10 @@ def main():
     roofline = None
     if bpl:
         ach = bpl / (avg_ms * 1e-3) / 1e9
+        config_key = f"E{plan.bit_edge_search_steps}-{'literal' if args.literal_edges else 'one-edge'}"
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, config_key),
+                    "kernel": dom,
                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bpl)}
     kernel_share = {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]}
 
@@ -136,8 +225,10 @@ def main():
                    "prns": len(prns), "samples_per_code": spc, "num_doppler_bins": ndop,
                    "num_blocks": nb, "fft_size": F, "num_coh": 10,
                    "bit_edge_search_steps": plan.bit_edge_search_steps, "num_noncoherent": nnc,
-                   "precision": "fp32", "signal": "device-resident int16"},
+                   "precision": "fp32", "signal": "device-resident int16",
+                   "edge_evaluation": "literal" if args.literal_edges else "one-edge identity"},
         "prn_acqs_per_s": round(len(prns) / dt, 2),
+        "host_signal_cells_per_s": round(cells / dt_host, 1),
         "cell_dwells_per_s": round(cells * nnc / dt, 1),
         "kernel_ms_per_step": kernel_share,
         "detected": det,

@@ -76,6 +76,13 @@ enum fmdbzp_precision {
     FMDBZP_PREC_FP32 = 0      /* plain FP32 path */
 };
 
+/* Plan options (fmdbzp_plan_desc.options). With N_db == 1 the bit-edge
+ * offsets of the sign search only rotate each column circularly, so every
+ * edge has the same |FFT|^2 (shift theorem) and the engine evaluates one edge;
+ * FMDBZP_OPT_LITERAL_EDGES evaluates every edge offset as the reference's loop
+ * does (src/noncoherent_integration.jl:670-684). Results agree to FP32 rounding. */
+#define FMDBZP_OPT_LITERAL_EDGES 0x1
+
 /* fmdbzp_acquire flags */
 #define FMDBZP_FLAG_STORE_SURFACES 0x1u  /* copy (ndop, spc_eff) power surfaces out */
 #define FMDBZP_FLAG_DEVICE_SIGNAL  0x2u  /* `signal` is a device pointer           */
@@ -104,7 +111,8 @@ typedef struct fmdbzp_plan_desc {
     int32_t noise_estimator;         /* enum fmdbzp_noise_estimator                 */
     int32_t samples_per_chip;        /* floor(fs / f_code), GlobalMean exclusion    */
     int32_t precision;               /* enum fmdbzp_precision                       */
-    int32_t reserved[7];
+    int32_t options;                 /* FMDBZP_OPT_* bits, 0 = defaults             */
+    int32_t reserved[6];
     double sampling_freq_hz;
 } fmdbzp_plan_desc;
 

@@ -101,18 +101,47 @@ def test_sign_path_two_data_bits_edge_search():
     assert is_detected(res[0])
 
 
-def test_sign_path_odd_doppler_bins():
-    """16.368 MHz, N_coh 5, E 5: ndop 55 (odd), the sign-path fold quirk."""
+@pytest.mark.parametrize("literal", [False, True])
+def test_sign_path_odd_doppler_bins(literal):
+    """16.368 MHz, N_coh 5, E 5: ndop 55 (odd), the sign-path fold quirk.
+    Run with the one-edge evaluation and with every edge evaluated."""
     sysm = GPSL1CA()
     fs = 16.368e6
     g = generate_test_signal(sysm, 5, num_samples=5 * 16368, sampling_freq=fs, interm_freq=0.0, CN0=45,
                              doppler=800.0, code_phase=512.25, seed=13)
-    plan, res, ores = run_case(sysm, fs, [5], g["signal"].astype(np.complex64), label="odd",
+    plan, res, ores = run_case(sysm, fs, [5], g["signal"].astype(np.complex64), label=f"odd literal={literal}",
                                min_doppler_coverage=5000.0, num_coherently_integrated_code_periods=5,
-                               bit_edge_search_steps=5)
+                               bit_edge_search_steps=5, literal_edges=literal)
     assert plan.num_doppler_bins % 2 == 1 and plan.path == 1
     # the sign-path fold never writes the last sorted row
     assert float(np.max(res[0].power_bins[-1, :])) == 0.0
+
+
+def test_one_edge_matches_literal_edges():
+    """Sign path with one data bit (N_coh 20, E 10, N_nc 2): the bit-edge
+    offsets are circular rotations of the column, so evaluating one edge gives
+    the surface of evaluating all ten, to float rounding."""
+    sysm = GPSL1CA()
+    fs = 2.048e6
+    g = generate_test_signal(sysm, 9, num_samples=2 * 20 * 2048, sampling_freq=fs, interm_freq=0.0, CN0=40,
+                             doppler=2100.0, code_phase=77.5, seed=21)
+    sig = g["signal"].astype(np.complex64)
+    surf = {}
+    for literal in (False, True):
+        plan = plan_acquire(sysm, fs, [9, 10], min_doppler_coverage=7000.0, num_coherently_integrated_code_periods=20,
+                            bit_edge_search_steps=10, num_noncoherent_accumulations=2, literal_edges=literal)
+        try:
+            assert plan.path == 1 and plan.num_data_bits == 1
+            res = acquire_(plan, sig, [9, 10], store_power_bins=True)
+            surf[literal] = [r.power_bins.copy() for r in res]
+            peaks = [(r.code_phase, r.carrier_doppler) for r in res]
+        finally:
+            plan.close()
+        surf[(literal, "peaks")] = peaks
+    for a, b in zip(surf[False], surf[True]):
+        rel = np.max(np.abs(a.astype(np.float64) - b)) / float(np.max(b))
+        assert rel < 1e-5, rel
+    assert surf[(False, "peaks")][0] == surf[(True, "peaks")][0]
 
 
 def test_rotation_streamed_l5i():
