@@ -304,7 +304,7 @@ __device__ void stage_io_generic(const In& in, const Out& out, int n, int R, int
     }
 }
 
-template <bool INV, bool COLFAST, class In, class Out>
+template <bool INV, bool COLFAST, bool R16, class In, class Out>
 __device__ __forceinline__ void stage_any(int R, const In& in, const Out& out, int n, int Ns, int nbatch,
                                           const float2* __restrict__ tw, int tid, int nthr)
 {
@@ -315,7 +315,7 @@ __device__ __forceinline__ void stage_any(int R, const In& in, const Out& out, i
     case 5: stage_io<5, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
     case 8: stage_io<8, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr); break;
     case 16:  // planned only for element-fast transforms
-        if constexpr (!COLFAST) stage_io<16, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr);
+        if constexpr (!COLFAST && R16) stage_io<16, INV, COLFAST>(in, out, n, Ns, nbatch, tw, tid, nthr);
         break;
     default: stage_io_generic<INV, COLFAST>(in, out, n, R, Ns, nbatch, tw, tid, nthr); break;
     }
@@ -324,7 +324,7 @@ __device__ __forceinline__ void stage_any(int R, const In& in, const Out& out, i
 // Whole transform through gin/gout. Every thread of the block must call it.
 // The caller must place a barrier before it if gin reads LDS written by other
 // threads, and after it before reading what gout wrote to LDS.
-template <bool INV, bool COLFAST, class GIn, class GOut>
+template <bool INV, bool COLFAST, class GIn, class GOut, bool R16 = true>
 __device__ void fft_run(float2* a, float2* b, const FftPlanDev& p, const float2* __restrict__ tw, int nbatch,
                         int bst, int tid, int nthr, const GIn& gin, const GOut& gout)
 {
@@ -334,16 +334,16 @@ __device__ void fft_run(float2* a, float2* b, const FftPlanDev& p, const float2*
         return;
     }
     if (p.nst == 1) {
-        stage_any<INV, COLFAST>(p.radix[0], gin, gout, n, 1, nbatch, tw, tid, nthr);
+        stage_any<INV, COLFAST, R16>(p.radix[0], gin, gout, n, 1, nbatch, tw, tid, nthr);
         return;
     }
-    stage_any<INV, COLFAST>(p.radix[0], gin, LdsOut<!COLFAST>{a, bst}, n, 1, nbatch, tw, tid, nthr);
+    stage_any<INV, COLFAST, R16>(p.radix[0], gin, LdsOut<!COLFAST>{a, bst}, n, 1, nbatch, tw, tid, nthr);
     __syncthreads();
     int Ns = p.radix[0];
     float2* src = a;
     float2* dst = b;
     for (int s = 1; s < p.nst - 1; s++) {
-        stage_any<INV, COLFAST>(p.radix[s], LdsIn<!COLFAST>{src, bst}, LdsOut<!COLFAST>{dst, bst}, n, Ns, nbatch, tw,
+        stage_any<INV, COLFAST, R16>(p.radix[s], LdsIn<!COLFAST>{src, bst}, LdsOut<!COLFAST>{dst, bst}, n, Ns, nbatch, tw,
                                 tid, nthr);
         __syncthreads();
         float2* t = src;
@@ -351,7 +351,7 @@ __device__ void fft_run(float2* a, float2* b, const FftPlanDev& p, const float2*
         dst = t;
         Ns *= p.radix[s];
     }
-    stage_any<INV, COLFAST>(p.radix[p.nst - 1], LdsIn<!COLFAST>{src, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
+    stage_any<INV, COLFAST, R16>(p.radix[p.nst - 1], LdsIn<!COLFAST>{src, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
 }
 
 // ---------------------------------------------------------------------------
@@ -362,9 +362,25 @@ __device__ void fft_run(float2* a, float2* b, const FftPlanDev& p, const float2*
 // barrier per intermediate stage. A thread holds at most MAXV complex values
 // per stage, so nbatch * n <= MAXV * nthr is required (checked on the host).
 // ---------------------------------------------------------------------------
+// Address-space-qualified pointers: the in-place stage is kept out of line
+// (inlining every radix into one kernel inflates its register count), and
+// these qualifiers keep its LDS and table accesses on ds_* / global_*
+// instructions instead of flat ones.
+typedef float fft_f2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) fft_f2v lds_float2;
+typedef __attribute__((address_space(1))) const fft_f2v gbl_float2;
+__device__ __forceinline__ float2 as_float2(fft_f2v v) { return make_float2(v.x, v.y); }
+__device__ __forceinline__ fft_f2v as_f2v(float2 v)
+{
+    fft_f2v r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+
 template <int R, bool INV, bool COLFAST, int MAXV>
-__device__ __noinline__ void stage_inplace(float2* a, int bst, int n, int Ns, int nbatch,
-                                              const float2* __restrict__ tw, int tid, int nthr)
+__device__ __noinline__ void stage_inplace(lds_float2* a, int bst, int n, int Ns, int nbatch,
+                                           gbl_float2* __restrict__ tw, int tid, int nthr)
 {
     constexpr int MB = (MAXV + R - 1) / R;
     const int m = n / R;
@@ -385,11 +401,15 @@ __device__ __noinline__ void stage_inplace(float2* a, int bst, int n, int Ns, in
             else t = fdivmod(idx, m, inv_b, &j);
             const int k = mod_ns(j, Ns);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = a[t * bst + lds_pad_if<!COLFAST>(j + r * m)];
+            for (int r = 0; r < R; r++) v[b][r] = as_float2(a[t * bst + lds_pad_if<!COLFAST>(j + r * m)]);
             if (Ns > 1) {
                 const int ts = k * tstride;
 #pragma unroll
-                for (int r = 1; r < R; r++) v[b][r] = cmulf(v[b][r], twid<INV>(tw, r * ts));
+                for (int r = 1; r < R; r++) {
+                    float2 w = as_float2(tw[r * ts]);
+                    if (INV) w.y = -w.y;
+                    v[b][r] = cmulf(v[b][r], w);
+                }
             }
             bfly<R, INV>(v[b]);
             ot[b] = t * bst;
@@ -401,7 +421,7 @@ __device__ __noinline__ void stage_inplace(float2* a, int bst, int n, int Ns, in
     for (int b = 0; b < MB; b++) {
         if (ot[b] >= 0) {
 #pragma unroll
-            for (int r = 0; r < R; r++) a[ot[b] + lds_pad_if<!COLFAST>(ob[b] + r * Ns)] = v[b][r];
+            for (int r = 0; r < R; r++) a[ot[b] + lds_pad_if<!COLFAST>(ob[b] + r * Ns)] = as_f2v(v[b][r]);
         }
     }
     __syncthreads();
@@ -420,14 +440,16 @@ template <bool INV, bool COLFAST, int MAXV>
 __device__ __forceinline__ void stage_inplace_any(int R, float2* a, int bst, int n, int Ns, int nbatch,
                                                   const float2* __restrict__ tw, int tid, int nthr)
 {
+    lds_float2* la = (lds_float2*)a;
+    gbl_float2* gt = (gbl_float2*)tw;
     switch (R) {
-    case 2: stage_inplace<2, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
-    case 3: stage_inplace<3, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
-    case 4: stage_inplace<4, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
-    case 5: stage_inplace<5, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
-    case 8: stage_inplace<8, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr); break;
+    case 2: stage_inplace<2, INV, COLFAST, MAXV>(la, bst, n, Ns, nbatch, gt, tid, nthr); break;
+    case 3: stage_inplace<3, INV, COLFAST, MAXV>(la, bst, n, Ns, nbatch, gt, tid, nthr); break;
+    case 4: stage_inplace<4, INV, COLFAST, MAXV>(la, bst, n, Ns, nbatch, gt, tid, nthr); break;
+    case 5: stage_inplace<5, INV, COLFAST, MAXV>(la, bst, n, Ns, nbatch, gt, tid, nthr); break;
+    case 8: stage_inplace<8, INV, COLFAST, MAXV>(la, bst, n, Ns, nbatch, gt, tid, nthr); break;
     default:  // 16, planned only for element-fast transforms
-        if constexpr (!COLFAST) stage_inplace<16, INV, COLFAST, MAXV>(a, bst, n, Ns, nbatch, tw, tid, nthr);
+        if constexpr (!COLFAST) stage_inplace<16, INV, COLFAST, MAXV>(la, bst, n, Ns, nbatch, gt, tid, nthr);
         break;
     }
 }
@@ -436,7 +458,7 @@ __device__ __forceinline__ void stage_inplace_any(int R, float2* a, int bst, int
 // Same contract as fft_run; additionally nbatch * n <= MAXV * nthr, and the
 // intermediate stages (all but the first and last) have radix 2, 3, 4, 5, 8
 // or 16 (fft_inplace_ok).
-template <bool INV, bool COLFAST, int MAXV, class GIn, class GOut>
+template <bool INV, bool COLFAST, int MAXV, class GIn, class GOut, bool R16 = true>
 __device__ void fft_run_1buf(float2* a, const FftPlanDev& p, const float2* __restrict__ tw, int nbatch, int bst,
                              int tid, int nthr, const GIn& gin, const GOut& gout)
 {
@@ -446,15 +468,15 @@ __device__ void fft_run_1buf(float2* a, const FftPlanDev& p, const float2* __res
         return;
     }
     if (p.nst == 1) {
-        stage_any<INV, COLFAST>(p.radix[0], gin, gout, n, 1, nbatch, tw, tid, nthr);
+        stage_any<INV, COLFAST, R16>(p.radix[0], gin, gout, n, 1, nbatch, tw, tid, nthr);
         return;
     }
-    stage_any<INV, COLFAST>(p.radix[0], gin, LdsOut<!COLFAST>{a, bst}, n, 1, nbatch, tw, tid, nthr);
+    stage_any<INV, COLFAST, R16>(p.radix[0], gin, LdsOut<!COLFAST>{a, bst}, n, 1, nbatch, tw, tid, nthr);
     __syncthreads();
     int Ns = p.radix[0];
     for (int s = 1; s < p.nst - 1; s++) {
         stage_inplace_any<INV, COLFAST, MAXV>(p.radix[s], a, bst, n, Ns, nbatch, tw, tid, nthr);
         Ns *= p.radix[s];
     }
-    stage_any<INV, COLFAST>(p.radix[p.nst - 1], LdsIn<!COLFAST>{a, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
+    stage_any<INV, COLFAST, R16>(p.radix[p.nst - 1], LdsIn<!COLFAST>{a, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
 }

@@ -119,6 +119,7 @@ struct fmdbzp_plan {
     int nprn = 0;
     int nsets = 0;
     FftPlanDev pf{}, pd{};
+    FftPlanDev pf8{};  // F-point plan without radix 16 (forward signal / code FFTs: fewer live inputs per thread)
     float2* d_twF = nullptr;
     float2* d_twD = nullptr;
     float2* d_prnconj = nullptr;
@@ -400,7 +401,8 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     };
     if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(FMDBZP_EHIP, "hipStreamCreate failed"));
-    if (make_fft_plan(d.fft_size, &P->pf, true) || make_fft_plan(d.num_doppler_bins, &P->pd, false))
+    if (make_fft_plan(d.fft_size, &P->pf, true) || make_fft_plan(d.fft_size, &P->pf8, false) ||
+        make_fft_plan(d.num_doppler_bins, &P->pd, false))
         return fail(set_err(FMDBZP_EINVAL, "FFT size has too many factors"));
     const int F = d.fft_size, ndop = d.num_doppler_bins, nb = d.num_blocks, bs = d.block_size;
     const int spc = d.samples_per_code, spc_eff = d.samples_per_code_eff;
@@ -512,7 +514,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         hipError_t e = hipMemcpy(d_codes, code_samples, (size_t)nprn * spc, hipMemcpyHostToDevice);
         if (e == hipSuccess) {
             Timed t(P, FMDBZP_K_CODE_FFT);
-            k_code_fft<<<dim3(nb, nprn), FMDBZP_NTHR, P->fft_lds, P->stream>>>(d_codes, P->d_prnconj, P->pf,
+            k_code_fft<<<dim3(nb, nprn), FMDBZP_NTHR, P->fft_lds, P->stream>>>(d_codes, P->d_prnconj, P->pf8,
                                                                               P->d_twF, spc, nb, bs);
             e = hipGetLastError();
         }
@@ -640,7 +642,7 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
         const float step = (float)((-twopi * interm_freq_hz) / d.sampling_freq_hz);
         Timed t(P, FMDBZP_K_SIGNAL_FFT);
         k_signal_fft<<<dim3(ndop, nnc), FMDBZP_NTHR, P->fft_lds, P->stream>>>(
-            d_sig, eltype, P->d_sigfft, P->pf, P->d_twF, bs, seglen, ndop, do_if, step);
+            d_sig, eltype, P->d_sigfft, P->pf8, P->d_twF, bs, seglen, ndop, do_if, step);
         HIP_TRY(hipGetLastError());
     }
 

@@ -106,8 +106,8 @@ k_code_fft(const int8_t* __restrict__ codes, float2* __restrict__ prnconj, FftPl
     const int8_t* code = codes + (size_t)prn * spc + (size_t)blk * bs;
     float2* out = prnconj + ((size_t)prn * nb + blk) * F;
     const int span = fft_lds_span(F);
-    fft_run<false, false>(lds, lds + span, pf, twF, 1, span, threadIdx.x, FMDBZP_NTHR, CodeIn{code, bs},
-                          ConjOut{out});
+    fft_run<false, false, CodeIn, ConjOut, false>(lds, lds + span, pf, twF, 1, span, threadIdx.x, FMDBZP_NTHR,
+                                                  CodeIn{code, bs}, ConjOut{out});
 }
 
 // ---------------------------------------------------------------------------
@@ -161,7 +161,8 @@ k_signal_fft(const void* __restrict__ sig, int eltype, float2* __restrict__ sigf
                   __fmul_rn((float)seg0, phase_step) /* Float32(seg_start-1) * step */, phase_step};
     float2* out = sigfft + ((size_t)step * ndop + gk) * F;
     const int span = fft_lds_span(F);
-    fft_run<false, false>(lds, lds + span, pf, twF, 1, span, threadIdx.x, FMDBZP_NTHR, in, PlainOut{out});
+    fft_run<false, false, DownconvIn, PlainOut, false>(lds, lds + span, pf, twF, 1, span, threadIdx.x, FMDBZP_NTHR,
+                                                       in, PlainOut{out});
 }
 
 // ---------------------------------------------------------------------------
