@@ -480,3 +480,90 @@ __device__ void fft_run_1buf(float2* a, const FftPlanDev& p, const float2* __res
     }
     stage_any<INV, COLFAST, R16>(p.radix[p.nst - 1], LdsIn<!COLFAST>{a, bst}, gout, n, Ns, nbatch, tw, tid, nthr);
 }
+
+// ---------------------------------------------------------------------------
+// Compile-time Stockham: transform size, radix sequence, batch and thread
+// count are template constants, so butterfly indices, twiddle strides and
+// padded LDS addresses fold to shifts and immediates. Element-fast mapping
+// (transform t = idx / (N / R)); NT transforms side by side, stride BST.
+// ---------------------------------------------------------------------------
+template <int R, bool INV, int N, int NS, int NT, int NTHR, class In, class Out>
+__device__ __forceinline__ void stage_static(const In& in, const Out& out, const float2* __restrict__ tw, int tid)
+{
+    constexpr int M = N / R;
+    constexpr int TOTAL = NT * M;
+    constexpr int TSTRIDE = N / (NS * R);
+    constexpr int ITERS = (TOTAL + NTHR - 1) / NTHR;
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const int idx = tid + it * NTHR;
+        if (TOTAL % NTHR != 0 && idx >= TOTAL) break;
+        const int t = idx / M, j = idx % M;
+        const int k = j % NS;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = in(t, j + r * M);
+        if (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmulf(v[r], twid<INV>(tw, r * k * TSTRIDE));
+        }
+        bfly<R, INV>(v);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) out(t, base + r * NS, v[r]);
+    }
+}
+
+// In place on one padded LDS buffer (transforms BST apart): read all inputs
+// of this thread's butterflies, barrier, compute and write, barrier.
+template <int R, bool INV, int N, int NS, int NT, int NTHR, int BST>
+__device__ __forceinline__ void stage_static_inplace(float2* a, const float2* __restrict__ tw, int tid)
+{
+    constexpr int M = N / R;
+    constexpr int TOTAL = NT * M;
+    constexpr int TSTRIDE = N / (NS * R);
+    constexpr int ITERS = (TOTAL + NTHR - 1) / NTHR;
+    float2 v[ITERS][R];
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const int idx = tid + it * NTHR;
+        if (TOTAL % NTHR == 0 || idx < TOTAL) {
+            const int t = idx / M, j = idx % M;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[it][r] = a[t * BST + lds_pad(j + r * M)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const int idx = tid + it * NTHR;
+        if (TOTAL % NTHR == 0 || idx < TOTAL) {
+            const int t = idx / M, j = idx % M;
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[it][r] = cmulf(v[it][r], twid<INV>(tw, r * k * TSTRIDE));
+            }
+            bfly<R, INV>(v[it]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) a[t * BST + lds_pad(base + r * NS)] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Three-stage transform R0 x R1 x R2 = N through gin / gout with one padded
+// LDS buffer per transform. Every thread of the block must call it; the
+// caller places a barrier before it when gin reads LDS written by others.
+template <bool INV, int N, int NT, int NTHR, int R0, int R1, int R2, class GIn, class GOut>
+__device__ __forceinline__ void fft3_static(float2* a, const float2* __restrict__ tw, int tid, const GIn& gin,
+                                            const GOut& gout)
+{
+    static_assert(R0 * R1 * R2 == N, "radix sequence must multiply to N");
+    constexpr int BST = N + (N >> 4) + 1;  // fft_lds_span(N)
+    stage_static<R0, INV, N, 1, NT, NTHR>(gin, LdsOut<true>{a, BST}, tw, tid);
+    __syncthreads();
+    stage_static_inplace<R1, INV, N, R0, NT, NTHR, BST>(a, tw, tid);
+    stage_static<R2, INV, N, R0 * R1, NT, NTHR>(LdsIn<true>{a, BST}, gout, tw, tid);
+}

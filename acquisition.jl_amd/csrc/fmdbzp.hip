@@ -152,7 +152,8 @@ struct fmdbzp_plan {
     bool simple_kernel = false;  // column stage runs k_column_simple
     int tp = 0, ap = 0;          // tile / accumulator row pitches (multiples of 32 elements)
     bool col_onebuf = false;     // simple column kernel runs its FFT stages in place
-    int corr_variant = 0;        // k_corr_tile: 2 = two in-place transforms per pass, 1 = one, 0 = ping-pong
+    int corr_variant = 0;        // k_corr_tile: 3 = compile-time F = 2048, 2 = two in-place transforms per
+                                 // pass, 1 = one, 0 = ping-pong
     size_t col_lds = 0;
     size_t fft_lds = 0;
     size_t corr_lds = 0;
@@ -416,7 +417,8 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // block correlation: the staged signal-FFT row plus, with in-place
     // stages, one buffer per concurrent transform (two when they fit the
     // register bound), else two ping-pong buffers
-    if (fft_inplace_ok(P->pf) && 2 * F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 2;
+    if (F == 2048) P->corr_variant = 3;  // compile-time 16 x 16 x 8 transforms, two per pass
+    else if (fft_inplace_ok(P->pf) && 2 * F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 2;
     else if (fft_inplace_ok(P->pf) && F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 1;
     else P->corr_variant = 0;
     P->corr_lds = (size_t)F * sizeof(float2) +
@@ -459,6 +461,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
 
     if ((rc = set_lds_attr((const void*)k_code_fft, P->fft_lds)) ||
         (rc = set_lds_attr((const void*)k_signal_fft, P->fft_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<2, true, 2048>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<2, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, false>, P->corr_lds)) ||
@@ -678,7 +681,10 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                 Timed t(P, FMDBZP_K_CORR_TILE);
                 const dim3 g(ndop, nbat);
                 const float2* sf = P->d_sigfft + (size_t)s * ndop * F;
-                if (P->corr_variant == 2)
+                if (P->corr_variant == 3)
+                    k_corr_tile<2, true, 2048><<<g, FMDBZP_NTHR, P->corr_lds, P->stream>>>(
+                        sf, P->d_prnconj, req, P->d_tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
+                else if (P->corr_variant == 2)
                     k_corr_tile<2, true><<<g, FMDBZP_NTHR, P->corr_lds, P->stream>>>(
                         sf, P->d_prnconj, req, P->d_tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
                 else if (P->corr_variant == 1)

@@ -188,9 +188,10 @@ struct LagOut {
     size_t tstride;     // tile distance between consecutive column blocks
     int bs;
     float inv_f;
+    int nt;             // transforms of this pass that exist (the rest are padding)
     __device__ __forceinline__ void operator()(int t, int i, float2 v) const
     {
-        if (i < bs) out[t * tstride + i] = make_float2(v.x * inv_f, v.y * inv_f);
+        if (i < bs && t < nt) out[t * tstride + i] = make_float2(v.x * inv_f, v.y * inv_f);
     }
 };
 
@@ -199,7 +200,8 @@ struct LagOut {
 // stages, one LDS buffer per concurrent transform; else two ping-pong buffers
 // (NT = 1).
 // grid (ndop, batch)
-template <int NT, bool ONEBUF>
+// STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass.
+template <int NT, bool ONEBUF, int STATIC = 0>
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
             const int32_t* __restrict__ req, float2* __restrict__ tile, FftPlanDev pf,
@@ -225,8 +227,10 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
         int b0 = kb + r0;
         if (b0 >= nb) b0 -= nb;
         CorrIn in{srow, Pp, F, nb, b0};
-        LagOut out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (size_t)ndop * tp, bs, inv_f};
-        if constexpr (ONEBUF)
+        LagOut out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (size_t)ndop * tp, bs, inv_f, nt};
+        if constexpr (STATIC == 2048)
+            fft3_static<true, 2048, NT, FMDBZP_NTHR, 16, 16, 8>(a, twF, tid, in, out);
+        else if constexpr (ONEBUF)
             fft_run_1buf<true, false, FMDBZP_CORR_MAXV>(a, pf, twF, nt, span, tid, FMDBZP_NTHR, in, out);
         else
             fft_run<true, false>(a, a + span, pf, twF, 1, span, tid, FMDBZP_NTHR, in, out);
