@@ -181,6 +181,30 @@ __device__ __forceinline__ void bfly16(float2* v)
     }
 }
 
+// 10-point DFT as 2 x 5: 5-point DFTs over the even and odd samples, twiddle
+// W10^c on the odd half, radix-2 combine; output k = c + 5 d.
+template <bool INV>
+__device__ __forceinline__ void bfly10(float2* v)
+{
+    const float c1 = 0.80901699437494742410f;  // cos(2 pi / 10)
+    const float s1 = 0.58778525229247312917f;  // sin(2 pi / 10)
+    const float c2 = 0.30901699437494742410f;  // cos(4 pi / 10)
+    const float s2 = 0.95105651629515357212f;  // sin(4 pi / 10)
+    float2 e[5] = {v[0], v[2], v[4], v[6], v[8]};
+    float2 o[5] = {v[1], v[3], v[5], v[7], v[9]};
+    bfly5<INV>(e);
+    bfly5<INV>(o);
+    o[1] = cmul_const<INV>(o[1], c1, -s1);   // W10^1
+    o[2] = cmul_const<INV>(o[2], c2, -s2);   // W10^2
+    o[3] = cmul_const<INV>(o[3], -c2, -s2);  // W10^3
+    o[4] = cmul_const<INV>(o[4], -c1, -s1);  // W10^4
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        v[c] = caddf(e[c], o[c]);
+        v[c + 5] = csubf(e[c], o[c]);
+    }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void bfly(float2* v)
 {
@@ -189,6 +213,7 @@ __device__ __forceinline__ void bfly(float2* v)
     else if constexpr (R == 4) bfly4<INV>(v);
     else if constexpr (R == 5) bfly5<INV>(v);
     else if constexpr (R == 8) bfly8<INV>(v);
+    else if constexpr (R == 10) bfly10<INV>(v);
     else bfly16<INV>(v);
 }
 
@@ -566,4 +591,47 @@ __device__ __forceinline__ void fft3_static(float2* a, const float2* __restrict_
     __syncthreads();
     stage_static_inplace<R1, INV, N, R0, NT, NTHR, BST>(a, tw, tid);
     stage_static<R2, INV, N, R0 * R1, NT, NTHR>(LdsIn<true>{a, BST}, gout, tw, tid);
+}
+
+// Column-fast compile-time stage: LB transforms (columns) of N points, lane
+// index = column fastest; columns >= ncol are skipped.
+template <int R, bool INV, int N, int NS, int LB, int NTHR, class In, class Out>
+__device__ __forceinline__ void stage_static_col(const In& in, const Out& out, const float2* __restrict__ tw, int tid,
+                                                 int ncol)
+{
+    constexpr int M = N / R;
+    constexpr int TOTAL = LB * M;
+    constexpr int TSTRIDE = N / (NS * R);
+    constexpr int ITERS = (TOTAL + NTHR - 1) / NTHR;
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const int idx = tid + it * NTHR;
+        if (TOTAL % NTHR != 0 && idx >= TOTAL) break;
+        const int t = idx % LB, j = idx / LB;
+        if (t >= ncol) continue;
+        const int k = j % NS;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = in(t, j + r * M);
+        if (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmulf(v[r], twid<INV>(tw, r * k * TSTRIDE));
+        }
+        bfly<R, INV>(v);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) out(t, base + r * NS, v[r]);
+    }
+}
+
+// Two-stage column-fast transform R0 x R1 = N (one LDS round trip, one
+// barrier) through gin / gout; column t lives at a + t * bst.
+template <bool INV, int N, int LB, int NTHR, int R0, int R1, class GIn, class GOut>
+__device__ __forceinline__ void fft2_static_col(float2* a, int bst, const float2* __restrict__ tw, int tid, int ncol,
+                                                const GIn& gin, const GOut& gout)
+{
+    static_assert(R0 * R1 == N, "radix pair must multiply to N");
+    stage_static_col<R0, INV, N, 1, LB, NTHR>(gin, LdsOut<false>{a, bst}, tw, tid, ncol);
+    __syncthreads();
+    stage_static_col<R1, INV, N, R0, LB, NTHR>(LdsIn<false>{a, bst}, gout, tw, tid, ncol);
 }

@@ -152,6 +152,7 @@ struct fmdbzp_plan {
     bool simple_kernel = false;  // column stage runs k_column_simple
     int tp = 0, ap = 0;          // tile / accumulator row pitches (multiples of 32 elements)
     bool col_onebuf = false;     // simple column kernel runs its FFT stages in place
+    bool col_static = false;     // simple column kernel uses the compile-time ndop = 160 transform
     int corr_variant = 0;        // k_corr_tile: 3 = compile-time F = 2048, 2 = two in-place transforms per
                                  // pass, 1 = one, 0 = ping-pong
     size_t col_lds = 0;
@@ -448,6 +449,9 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     const size_t budget = P->simple_kernel ? kLdsDefault : kLdsDefault - 1024;
     int LB = budget > fixed ? (int)((budget - fixed) / per_col) : 0;
     if (P->col_onebuf && LB > FMDBZP_COL_MAXV * FMDBZP_COL_NTHR / ndop) LB = FMDBZP_COL_MAXV * FMDBZP_COL_NTHR / ndop;
+    // ndop 160: compile-time 10 x 16 transform over a fixed column count
+    P->col_static = P->simple_kernel && ndop == 160;
+    if (P->col_static) LB = FMDBZP_COL_STATIC_LB;
     if (LB > 64) LB = 64;
     // whole 128-byte accumulator / 256-byte tile segments where possible
     if (LB >= 32) LB &= ~31;
@@ -465,6 +469,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         (rc = set_lds_attr((const void*)k_corr_tile<2, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, false>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_simple<true, 160>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_simple<true>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_simple<false>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_sign, P->col_lds)) ||
@@ -702,7 +707,9 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                 dim3 g(nlb, nb, nbat);
                 if (P->simple_kernel)
                 {
-                    if (P->col_onebuf)
+                    if (P->col_static)
+                        k_column_simple<true, 160><<<g, FMDBZP_COL_NTHR, P->col_lds, P->stream>>>(A);
+                    else if (P->col_onebuf)
                         k_column_simple<true><<<g, FMDBZP_COL_NTHR, P->col_lds, P->stream>>>(A);
                     else
                         k_column_simple<false><<<g, FMDBZP_COL_NTHR, P->col_lds, P->stream>>>(A);

@@ -289,7 +289,10 @@ struct SimpleAccOut {
 
 // `_accumulate_fftshifted_power_(drift_)pilot!` (src/noncoherent_integration.jl:482-592)
 // and `_stream_power_column!` (src/acquire.jl:335-385).
-template <bool ONEBUF>
+// STATIC = 160: compile-time 10 x 16 column transform (ndop 160) over
+// FMDBZP_COL_STATIC_LB columns, one LDS round trip.
+#define FMDBZP_COL_STATIC_LB 32
+template <bool ONEBUF, int STATIC = 0>
 __global__ void __launch_bounds__(FMDBZP_COL_NTHR, 4)
 k_column_simple(ColumnArgs A)
 {
@@ -301,7 +304,10 @@ k_column_simple(ColumnArgs A)
     const int bst = A.bstride;
     const float2* T = A.tile + (((size_t)p * A.nb + r) * A.ndop) * A.tp + lag0;
     SimpleAccOut out{&A, p, r * A.bs + lag0, A.ndop / 2};
-    if constexpr (ONEBUF)
+    if constexpr (STATIC == 160)
+        fft2_static_col<false, 160, FMDBZP_COL_STATIC_LB, FMDBZP_COL_NTHR, 10, 16>(lds, bst, A.twD, threadIdx.x, ncol,
+                                                                                 TileColIn{T, A.tp}, out);
+    else if constexpr (ONEBUF)
         fft_run_1buf<false, true, FMDBZP_COL_MAXV>(lds, A.pd, A.twD, ncol, bst, threadIdx.x, FMDBZP_COL_NTHR,
                                                     TileColIn{T, A.tp}, out);
     else
