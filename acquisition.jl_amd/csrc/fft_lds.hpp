@@ -592,46 +592,3 @@ __device__ __forceinline__ void fft3_static(float2* a, const float2* __restrict_
     stage_static_inplace<R1, INV, N, R0, NT, NTHR, BST>(a, tw, tid);
     stage_static<R2, INV, N, R0 * R1, NT, NTHR>(LdsIn<true>{a, BST}, gout, tw, tid);
 }
-
-// Column-fast compile-time stage: LB transforms (columns) of N points, lane
-// index = column fastest; columns >= ncol are skipped.
-template <int R, bool INV, int N, int NS, int LB, int NTHR, class In, class Out>
-__device__ __forceinline__ void stage_static_col(const In& in, const Out& out, const float2* __restrict__ tw, int tid,
-                                                 int ncol)
-{
-    constexpr int M = N / R;
-    constexpr int TOTAL = LB * M;
-    constexpr int TSTRIDE = N / (NS * R);
-    constexpr int ITERS = (TOTAL + NTHR - 1) / NTHR;
-#pragma unroll
-    for (int it = 0; it < ITERS; it++) {
-        const int idx = tid + it * NTHR;
-        if (TOTAL % NTHR != 0 && idx >= TOTAL) break;
-        const int t = idx % LB, j = idx / LB;
-        if (t >= ncol) continue;
-        const int k = j % NS;
-        float2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) v[r] = in(t, j + r * M);
-        if (NS > 1) {
-#pragma unroll
-            for (int r = 1; r < R; r++) v[r] = cmulf(v[r], twid<INV>(tw, r * k * TSTRIDE));
-        }
-        bfly<R, INV>(v);
-        const int base = (j - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; r++) out(t, base + r * NS, v[r]);
-    }
-}
-
-// Two-stage column-fast transform R0 x R1 = N (one LDS round trip, one
-// barrier) through gin / gout; column t lives at a + t * bst.
-template <bool INV, int N, int LB, int NTHR, int R0, int R1, class GIn, class GOut>
-__device__ __forceinline__ void fft2_static_col(float2* a, int bst, const float2* __restrict__ tw, int tid, int ncol,
-                                                const GIn& gin, const GOut& gout)
-{
-    static_assert(R0 * R1 == N, "radix pair must multiply to N");
-    stage_static_col<R0, INV, N, 1, LB, NTHR>(gin, LdsOut<false>{a, bst}, tw, tid, ncol);
-    __syncthreads();
-    stage_static_col<R1, INV, N, R0, LB, NTHR>(LdsIn<false>{a, bst}, gout, tw, tid, ncol);
-}

@@ -450,8 +450,6 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     int LB = budget > fixed ? (int)((budget - fixed) / per_col) : 0;
     if (P->col_onebuf && LB > FMDBZP_COL_MAXV * FMDBZP_COL_NTHR / ndop) LB = FMDBZP_COL_MAXV * FMDBZP_COL_NTHR / ndop;
     // ndop 160: compile-time 10 x 16 transform over a fixed column count
-    P->col_static = P->simple_kernel && ndop == 160;
-    if (P->col_static) LB = FMDBZP_COL_STATIC_LB;
     if (LB > 64) LB = 64;
     // whole 128-byte accumulator / 256-byte tile segments where possible
     if (LB >= 32) LB &= ~31;
@@ -459,6 +457,10 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     else if (LB >= 8) LB &= ~7;
     if (LB < 1) LB = 1;
     if (LB > bs) LB = bs;
+    // ndop 160 (and a column block of at least 32 lags): compile-time 10 x 16
+    // column kernel with a fixed column count (k_column_c160)
+    P->col_static = P->simple_kernel && ndop == 160 && bs >= 32;
+    if (P->col_static) LB = 32;
     P->LB = LB;
     P->col_lds = per_col * LB + fixed;
     if (P->col_lds > kLdsMax) return fail(set_err(FMDBZP_EINVAL, "column stage does not fit LDS"));
@@ -469,7 +471,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         (rc = set_lds_attr((const void*)k_corr_tile<2, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, false>, P->corr_lds)) ||
-        (rc = set_lds_attr((const void*)k_column_simple<true, 160>, P->col_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_c160<32, 512>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_simple<true>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_simple<false>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_sign, P->col_lds)) ||
@@ -708,7 +710,7 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                 if (P->simple_kernel)
                 {
                     if (P->col_static)
-                        k_column_simple<true, 160><<<g, FMDBZP_COL_NTHR, P->col_lds, P->stream>>>(A);
+                        k_column_c160<32, 512><<<g, 512, P->col_lds, P->stream>>>(A);
                     else if (P->col_onebuf)
                         k_column_simple<true><<<g, FMDBZP_COL_NTHR, P->col_lds, P->stream>>>(A);
                     else

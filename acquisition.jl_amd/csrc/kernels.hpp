@@ -289,10 +289,7 @@ struct SimpleAccOut {
 
 // `_accumulate_fftshifted_power_(drift_)pilot!` (src/noncoherent_integration.jl:482-592)
 // and `_stream_power_column!` (src/acquire.jl:335-385).
-// STATIC = 160: compile-time 10 x 16 column transform (ndop 160) over
-// FMDBZP_COL_STATIC_LB columns, one LDS round trip.
-#define FMDBZP_COL_STATIC_LB 32
-template <bool ONEBUF, int STATIC = 0>
+template <bool ONEBUF>
 __global__ void __launch_bounds__(FMDBZP_COL_NTHR, 4)
 k_column_simple(ColumnArgs A)
 {
@@ -304,15 +301,85 @@ k_column_simple(ColumnArgs A)
     const int bst = A.bstride;
     const float2* T = A.tile + (((size_t)p * A.nb + r) * A.ndop) * A.tp + lag0;
     SimpleAccOut out{&A, p, r * A.bs + lag0, A.ndop / 2};
-    if constexpr (STATIC == 160)
-        fft2_static_col<false, 160, FMDBZP_COL_STATIC_LB, FMDBZP_COL_NTHR, 10, 16>(lds, bst, A.twD, threadIdx.x, ncol,
-                                                                                 TileColIn{T, A.tp}, out);
-    else if constexpr (ONEBUF)
+    if constexpr (ONEBUF)
         fft_run_1buf<false, true, FMDBZP_COL_MAXV>(lds, A.pd, A.twD, ncol, bst, threadIdx.x, FMDBZP_COL_NTHR,
                                                     TileColIn{T, A.tp}, out);
     else
         fft_run<false, true>(lds, lds + (size_t)A.LB * bst, A.pd, A.twD, ncol, bst, threadIdx.x, FMDBZP_COL_NTHR,
                              TileColIn{T, A.tp}, out);
+}
+
+// Simple column stage for ndop = 160 with compile-time 10 x 16 transforms:
+// LBS columns per workgroup of NT threads. Each stage-2 thread owns one
+// radix-16 butterfly of one column (16 accumulator cells); it issues the loads
+// of those cells before the tile loads, so both reads are in flight together
+// and the accumulate needs no second memory round trip.
+template <int LBS, int NT>
+__global__ void __launch_bounds__(NT)
+k_column_c160(ColumnArgs A)
+{
+    constexpr int N = 160, R0 = 10, M0 = N / R0, R1 = 16, M1 = N / R1;
+    extern __shared__ float2 lds[];
+    const int lbk = xcd_logical_block();
+    const int lb = lbk % gridDim.x, r = (lbk / gridDim.x) % gridDim.y, p = lbk / (gridDim.x * gridDim.y);
+    const int lag0 = lb * LBS;
+    const int ncol = min(LBS, A.bs - lag0);
+    const int bst = A.bstride;
+    const int tid = threadIdx.x;
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * N) * A.tp + lag0;
+    float* accp = A.acc + (size_t)p * N * A.ap;
+
+    // stage-2 role: column t1, butterfly j1, outputs w = j1 + 10 q -> row (w + 80) mod 160
+    const int t1 = tid % LBS, j1 = tid / LBS;
+    const bool act1 = tid < LBS * M1 && t1 < ncol;
+    const int cp = r * A.bs + lag0 + t1;
+    float accv[R1];
+    if (act1 && !A.first_step) {
+#pragma unroll
+        for (int q = 0; q < R1; q++) {
+            int row = j1 + M1 * q + N / 2;
+            if (row >= N) row -= N;
+            int dc = cp + A.shifts[row];
+            if (dc >= A.spc) dc -= A.spc;
+            accv[q] = accp[(size_t)row * A.ap + dc];
+        }
+    }
+
+    // stage 1: radix 10 over rows j + 16 m, straight from the tile
+#pragma unroll
+    for (int it = 0; it < (LBS * M0 + NT - 1) / NT; it++) {
+        const int idx = tid + it * NT;
+        if (idx < LBS * M0) {
+            const int t = idx % LBS, j = idx / LBS;
+            if (t < ncol) {
+                float2 v[R0];
+#pragma unroll
+                for (int m = 0; m < R0; m++) v[m] = T[(size_t)(j + M0 * m) * A.tp + t];
+                bfly10<false>(v);
+#pragma unroll
+                for (int m = 0; m < R0; m++) lds[t * bst + j * R0 + m] = v[m];
+            }
+        }
+    }
+    __syncthreads();
+    // stage 2: twiddle W_160^(q j1), radix 16, power, fftshift fold, drift, accumulate
+    if (act1) {
+        float2 v[R1];
+#pragma unroll
+        for (int q = 0; q < R1; q++) v[q] = lds[t1 * bst + j1 + M1 * q];
+#pragma unroll
+        for (int q = 1; q < R1; q++) v[q] = cmulf(v[q], A.twD[q * j1]);
+        bfly16<false>(v);
+#pragma unroll
+        for (int q = 0; q < R1; q++) {
+            int row = j1 + M1 * q + N / 2;
+            if (row >= N) row -= N;
+            int dc = cp + A.shifts[row];
+            if (dc >= A.spc) dc -= A.spc;
+            const float pw = abs2_exact(v[q]);
+            accp[(size_t)row * A.ap + dc] = A.first_step ? pw : accv[q] + pw;
+        }
+    }
 }
 
 // Power of each raw bin into an LDS float array (stride sst per column).
