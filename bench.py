@@ -30,6 +30,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def hbm_stream(nbytes: int):
+    """Copy, read-only and write-only streams of libhbmcopy.so over `nbytes`:
+    the HBM bandwidth this device reaches in practice (the roofline peak stays
+    the 8 TB/s datasheet figure)."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "acquisition.jl_amd", "acquisition", "_lib", "libhbmcopy.so"))
+    lib.hbm_stream_run.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+    lib.hbm_stream_run.restype = ctypes.c_double
+    out = {"bytes": nbytes}
+    for kind, name, moved in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
+        ms = lib.hbm_stream_run(nbytes, 10, kind)
+        if ms <= 0:
+            raise SystemExit(f"hbm stream {name} failed ({ms})")
+        out[f"{name}_gb_per_s"] = round(moved * nbytes / (ms * 1e-3) / 1e9, 1)
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(nbytes: int):
     """Time the plain copy kernel of libhbmcopy.so. Profiled with the
     FETCH_SIZE / WRITE_SIZE counters, its per-dispatch readings over
@@ -47,7 +64,8 @@ def copy_calibration(nbytes: int):
 
 
 KERNEL_CLASS_OF = {"k_corr_tile": "corr_tile", "k_column_simple": "column", "k_column_sign": "column",
-                   "k_column_rotation": "column", "k_signal_fft": "signal_fft", "k_reduce_rows": "reduce",
+                   "k_column_rotation": "column", "k_column_c160": "column", "k_signal_fft": "signal_fft",
+                   "k_reduce_rows": "reduce",
                    "k_finalize": "finalize", "k_code_fft": "code_fft"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_l1ca_16mhz_32prn.json")
 
@@ -113,7 +131,12 @@ def main():
                     help="only run a plain device copy of MIB MiB (FETCH_SIZE/WRITE_SIZE counter calibration)")
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="only summarise rocprofv3 FETCH_SIZE/WRITE_SIZE CSV runs under DIR")
+    ap.add_argument("--hbm-stream", type=float, default=0.0, metavar="MIB",
+                    help="only measure copy / read / write streaming bandwidth over MIB MiB buffers")
     args = ap.parse_args()
+    if args.hbm_stream > 0:
+        hbm_stream(int(args.hbm_stream * 2**20))
+        return
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, f"E{args.edge_steps}-{'literal' if args.literal_edges else 'one-edge'}")
         return
