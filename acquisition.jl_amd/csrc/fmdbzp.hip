@@ -153,6 +153,15 @@ struct fmdbzp_plan {
     double* d_rowsum = nullptr;
     float* d_rowbest_v = nullptr;
     uint64_t* d_rowbest_k = nullptr;
+    // column-major accumulator (few columns per column-stage workgroup): the
+    // row reduction runs over column chunks of red_cw, partials per chunk
+    bool acc_cm = false;
+    int apc = 0;             // column pitch (ndop rounded up to 32)
+    size_t acc_ps = 0;       // accumulator elements per PRN
+    int red_cw = 0, red_nchunk = 0;
+    double* d_psum = nullptr;
+    float* d_pbest_v = nullptr;
+    uint64_t* d_pbest_k = nullptr;
     int32_t* d_req = nullptr;
     size_t req_cap = 0;
     fmdbzp_prn_stats* d_stats = nullptr;
@@ -338,6 +347,9 @@ void free_plan(fmdbzp_plan* P)
     hipFree(P->d_rowsum);
     hipFree(P->d_rowbest_v);
     hipFree(P->d_rowbest_k);
+    hipFree(P->d_psum);
+    hipFree(P->d_pbest_v);
+    hipFree(P->d_pbest_k);
     hipFree(P->d_req);
     hipFree(P->d_stats);
     hipFree(P->d_export);
@@ -495,6 +507,14 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     P->col_static = P->simple_kernel && ndop == 160 && bs >= 32;
     if (P->col_static) LB = 32;
     P->LB = LB;
+    // With fewer than 16 columns per workgroup, a row-major accumulator turns
+    // every scatter into isolated 4-byte updates; those plans keep the
+    // accumulator column-major, with rows contiguous per code column.
+    P->acc_cm = !P->simple_kernel && LB < 16;
+    P->apc = (ndop + 31) & ~31;
+    P->acc_ps = P->acc_cm ? (size_t)spc_eff * P->apc : (size_t)ndop * P->ap;
+    P->red_cw = 1024;
+    P->red_nchunk = (spc_eff + P->red_cw - 1) / P->red_cw;
     P->col_lds = per_col * LB + fixed;
     if (P->col_lds > kLdsMax) return fail(set_err(FMDBZP_EINVAL, "column stage does not fit LDS"));
 
@@ -569,7 +589,8 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // workspace: signal FFT cache for all segments + PRN-batch tile/accumulator
     if ((rc = dev_alloc(&P->d_sigfft, (size_t)d.num_noncoherent * ndop * F))) return fail(rc);
     {
-        const size_t per_prn = (size_t)nb * ndop * P->tp * sizeof(float2) + (size_t)ndop * P->ap * sizeof(float);
+        const size_t per_prn = (size_t)nb * ndop * P->tp * sizeof(float2) + P->acc_ps * sizeof(float) +
+                               (P->acc_cm ? (size_t)P->red_nchunk * ndop * 20 : 0);
         size_t budget = workspace_budget();
         size_t b = budget / per_prn;
         if (b < 1) b = 1;
@@ -579,11 +600,17 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         P->nlanes = P->batch >= lanes_requested() ? lanes_requested() : 1;
     }
     if ((rc = dev_alloc(&P->d_tile, (size_t)P->batch * nb * ndop * P->tp)) ||
-        (rc = dev_alloc(&P->d_acc, (size_t)P->batch * ndop * P->ap)) ||
+        (rc = dev_alloc(&P->d_acc, (size_t)P->batch * P->acc_ps)) ||
         (rc = dev_alloc(&P->d_rowsum, (size_t)P->batch * ndop)) ||
         (rc = dev_alloc(&P->d_rowbest_v, (size_t)P->batch * ndop)) ||
         (rc = dev_alloc(&P->d_rowbest_k, (size_t)P->batch * ndop)))
         return fail(rc);
+    if (P->acc_cm) {
+        const size_t np = (size_t)P->batch * P->red_nchunk * ndop;
+        if ((rc = dev_alloc(&P->d_psum, np)) || (rc = dev_alloc(&P->d_pbest_v, np)) ||
+            (rc = dev_alloc(&P->d_pbest_k, np)))
+            return fail(rc);
+    }
     *out_plan = P;
     return FMDBZP_OK;
 }
@@ -708,6 +735,9 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
     A.literal_edges = (d.options & FMDBZP_OPT_LITERAL_EDGES) ? 1 : 0;
     A.tp = P->tp;
     A.ap = P->ap;
+    A.acc_ps = P->acc_ps;
+    A.acc_ldr = P->acc_cm ? 1 : P->ap;
+    A.acc_ldc = P->acc_cm ? P->apc : 1;
     const int streamed_rot = (nnc == 1 && d.path == FMDBZP_PATH_ROTATION) ? 1 : 0;
     const int nlb = (bs + P->LB - 1) / P->LB;
 
@@ -715,7 +745,7 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
     HIP_TRY(hipEventRecord(P->ev_fork, P->stream));
     for (int l = 0; l < P->nlanes; l++) HIP_TRY(hipStreamWaitEvent(P->lane[l], P->ev_fork, 0));
     const size_t tile_per_prn = (size_t)nb * ndop * P->tp;
-    const size_t acc_per_prn = (size_t)ndop * P->ap;
+    const size_t acc_per_prn = P->acc_ps;
     const bool dev_out = (flags & FMDBZP_FLAG_DEVICE_OUTPUT) != 0;
     for (int b0 = 0; b0 < nreq; b0 += P->batch) {
         const int nbat = nreq - b0 < P->batch ? nreq - b0 : P->batch;
@@ -776,15 +806,28 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
             }
             {
                 Timed t(P, FMDBZP_K_REDUCE, st);
-                k_reduce_rows<<<dim3(ndop, cnt), FMDBZP_NTHR, 0, st>>>(acc, ndop, spc, spc_eff, P->ap,
-                                                                       d.num_secondary_rotations, streamed_rot,
+                if (P->acc_cm) {
+                    const size_t po = (size_t)lo * P->red_nchunk * ndop;
+                    const int rb = (ndop + FMDBZP_NTHR - 1) / FMDBZP_NTHR;
+                    k_reduce_cols<<<dim3(P->red_nchunk, rb, cnt), FMDBZP_NTHR, 0, st>>>(
+                        acc, P->acc_ps, P->apc, ndop, spc, spc_eff, d.num_secondary_rotations, streamed_rot,
+                        P->red_cw, P->d_psum + po, P->d_pbest_v + po, P->d_pbest_k + po);
+                    HIP_TRY(hipGetLastError());
+                    k_merge_rows<<<dim3(rb, cnt), FMDBZP_NTHR, 0, st>>>(P->d_psum + po, P->d_pbest_v + po,
+                                                                       P->d_pbest_k + po, P->red_nchunk, ndop,
                                                                        rowsum, rowbest_v, rowbest_k);
+                } else {
+                    k_reduce_rows<<<dim3(ndop, cnt), FMDBZP_NTHR, 0, st>>>(acc, ndop, spc, spc_eff, P->ap,
+                                                                           d.num_secondary_rotations, streamed_rot,
+                                                                           rowsum, rowbest_v, rowbest_k);
+                }
                 HIP_TRY(hipGetLastError());
             }
             {
                 Timed t(P, FMDBZP_K_FINALIZE, st);
                 k_finalize<<<dim3(cnt), FMDBZP_NTHR, 0, st>>>(acc, rowsum, rowbest_v, rowbest_k, ndop, spc, spc_eff,
-                                                              P->ap, d.num_secondary_rotations, streamed_rot,
+                                                              P->acc_ps, A.acc_ldr, A.acc_ldc,
+                                                              d.num_secondary_rotations, streamed_rot,
                                                               d.noise_estimator, d.samples_per_chip,
                                                               P->d_stats + b0 + lo);
                 HIP_TRY(hipGetLastError());
@@ -796,9 +839,15 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                     if (!dst) continue;
                     {
                         Timed t(P, FMDBZP_K_EXPORT, st);
-                        dim3 g((spc_eff + 31) / 32, (ndop + 31) / 32);
-                        k_export<<<g, dim3(32, 8), 0, st>>>(acc + (size_t)p * acc_per_prn, dev_out ? dst : stage,
-                                                            ndop, spc_eff, P->ap);
+                        if (P->acc_cm) {
+                            dim3 g(spc_eff, (ndop + FMDBZP_NTHR - 1) / FMDBZP_NTHR);
+                            k_export_cm<<<g, FMDBZP_NTHR, 0, st>>>(acc + (size_t)p * acc_per_prn,
+                                                                   dev_out ? dst : stage, ndop, P->apc);
+                        } else {
+                            dim3 g((spc_eff + 31) / 32, (ndop + 31) / 32);
+                            k_export<<<g, dim3(32, 8), 0, st>>>(acc + (size_t)p * acc_per_prn,
+                                                                dev_out ? dst : stage, ndop, spc_eff, P->ap);
+                        }
                         HIP_TRY(hipGetLastError());
                     }
                     if (!dev_out)

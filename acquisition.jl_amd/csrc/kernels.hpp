@@ -254,6 +254,10 @@ struct ColumnArgs {
     int nb, bs, ndop, spc, spc_eff, ncoh, ndb, E, R, Q, ndc, edge_step, use_max;
     int LB, bstride, sst, first_step, literal_edges;
     int tp, ap;  // tile and accumulator row pitches (elements, multiples of 32)
+    // accumulator element (p, row, col) at acc + p * acc_ps + row * acc_ldr + col * acc_ldc:
+    // row-major (acc_ldr = ap, acc_ldc = 1) or column-major (acc_ldr = 1, acc_ldc = column pitch)
+    size_t acc_ps;
+    int acc_ldr, acc_ldc;
 };
 
 // Accumulate (or store, at the first step) one power value with the drift
@@ -263,7 +267,7 @@ __device__ __forceinline__ void acc_cell(const ColumnArgs& A, int p, int row, in
 {
     int dc = cp + A.shifts[row];
     if (dc >= A.spc) dc -= A.spc;
-    float* dst = A.acc + ((size_t)p * A.ndop + row) * A.ap + (size_t)slice * A.spc + dc;
+    float* dst = A.acc + (size_t)p * A.acc_ps + (size_t)row * A.acc_ldr + ((size_t)slice * A.spc + dc) * A.acc_ldc;
     if (A.first_step) *dst = v;
     else *dst = *dst + v;
 }
@@ -423,9 +427,20 @@ __device__ __forceinline__ void fold_into_stage(float* stage, const float* pw, i
 
 // Scatter a stage (sorted rows) into the accumulator with drift
 // (`_accumulate_sign_tile!` src/noncoherent_integration.jl:695-746).
+// Lanes run along columns for a row-major accumulator and along rows for a
+// column-major one, so each store instruction covers contiguous cells.
 __device__ __forceinline__ void scatter_stage(const ColumnArgs& A, const float* stage, int p, int r,
                                               int lag0, int ncol, int slice, int tid)
 {
+    if (A.acc_ldr == 1) {
+        const float inv = 1.0f / (float)A.ndop;
+        for (int idx = tid; idx < ncol * A.ndop; idx += FMDBZP_NTHR) {
+            int row;
+            const int c = fdivmod(idx, A.ndop, inv, &row);
+            acc_cell(A, p, row, r * A.bs + lag0 + c, slice, stage[c * A.sst + row]);
+        }
+        return;
+    }
     const float inv = 1.0f / (float)ncol;
     for (int idx = tid; idx < ncol * A.ndop; idx += FMDBZP_NTHR) {
         int c;
@@ -654,13 +669,63 @@ k_reduce_rows(const float* __restrict__ acc, int ndop, int spc, int spc_eff, int
     }
 }
 
+// Column-major accumulator, pass 1. grid (column chunks, ceil(ndop / NTHR),
+// batch): each thread owns one row and walks the chunk's `cw` columns in
+// order (consecutive lanes read consecutive rows of a column), producing the
+// chunk's FP64 row sum and best (value, visit-order key).
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_reduce_cols(const float* __restrict__ acc, size_t acc_ps, int ldc, int ndop, int spc, int spc_eff, int R,
+              int streamed_rot, int cw, double* __restrict__ psum, float* __restrict__ pbest_v,
+              uint64_t* __restrict__ pbest_k)
+{
+    const int chunk = blockIdx.x, row = blockIdx.y * FMDBZP_NTHR + threadIdx.x, p = blockIdx.z;
+    if (row >= ndop) return;
+    const float* a = acc + (size_t)p * acc_ps + row;
+    const int c0 = chunk * cw, c1 = min(spc_eff, c0 + cw);
+    double s = 0.0;
+    float bv = -INFINITY;
+    uint64_t bk = ~0ull;
+    for (int c = c0; c < c1; c++) {
+        const float v = a[(size_t)c * ldc];
+        s += (double)v;
+        const uint64_t k = cell_key(row, c, ndop, spc, R, streamed_rot);
+        if (better(v, k, bv, bk)) { bv = v; bk = k; }
+    }
+    const size_t o = ((size_t)p * gridDim.x + chunk) * ndop + row;
+    psum[o] = s;
+    pbest_v[o] = bv;
+    pbest_k[o] = bk;
+}
+
+// Column-major accumulator, pass 2. grid (ceil(ndop / NTHR), batch): per row,
+// the chunk partials in chunk order give the row sum and the row's best cell.
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_merge_rows(const double* __restrict__ psum, const float* __restrict__ pbest_v,
+             const uint64_t* __restrict__ pbest_k, int nchunk, int ndop, double* __restrict__ rowsum,
+             float* __restrict__ rowbest_v, uint64_t* __restrict__ rowbest_k)
+{
+    const int row = blockIdx.x * FMDBZP_NTHR + threadIdx.x, p = blockIdx.y;
+    if (row >= ndop) return;
+    double s = 0.0;
+    float bv = -INFINITY;
+    uint64_t bk = ~0ull;
+    for (int c = 0; c < nchunk; c++) {
+        const size_t o = ((size_t)p * nchunk + c) * ndop + row;
+        s += psum[o];
+        if (better(pbest_v[o], pbest_k[o], bv, bk)) { bv = pbest_v[o]; bk = pbest_k[o]; }
+    }
+    rowsum[(size_t)p * ndop + row] = s;
+    rowbest_v[(size_t)p * ndop + row] = bv;
+    rowbest_k[(size_t)p * ndop + row] = bk;
+}
+
 // grid (batch): peak, noise estimate (src/acquire.jl:687-697,
 // src/est_signal_noise_power.jl:211-243) and interpolation neighbours.
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
            const float* __restrict__ rowbest_v, const uint64_t* __restrict__ rowbest_k, int ndop,
-           int spc, int spc_eff, int ap, int R, int streamed_rot, int noise_est, int spchip,
-           fmdbzp_prn_stats* __restrict__ stats)
+           int spc, int spc_eff, size_t acc_ps, int ldr, int ldc, int R, int streamed_rot, int noise_est,
+           int spchip, fmdbzp_prn_stats* __restrict__ stats)
 {
     __shared__ double ssum[FMDBZP_NTHR];
     __shared__ float sv[FMDBZP_NTHR];
@@ -715,7 +780,7 @@ k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
     }
     __syncthreads();
     const int row = s_row, col = s_col;
-    const float* a = acc + (size_t)p * ndop * ap;
+    const float* a = acc + (size_t)p * acc_ps;
     // GlobalMean exclusion: columns [col - spchip, col + spchip] clipped, all rows
     int lo = col - spchip;
     if (lo < 0) lo = 0;
@@ -726,7 +791,7 @@ k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
     if (noise_est == FMDBZP_NOISE_GLOBAL_MEAN) {
         for (int idx = tid; idx < nex * ndop; idx += FMDBZP_NTHR) {
             const int rr = idx / nex, cc = lo + idx - rr * nex;
-            ex += (double)a[(size_t)rr * ap + cc];
+            ex += (double)a[(size_t)rr * ldr + (size_t)cc * ldc];
         }
     }
     __syncthreads();
@@ -759,12 +824,22 @@ k_finalize(const float* __restrict__ acc, const double* __restrict__ rowsum,
         const int cr = (scr == spc - 1 ? 0 : scr + 1) + rot * spc;
         const int dl = row == 0 ? ndop - 1 : row - 1;
         const int dr = row == ndop - 1 ? 0 : row + 1;
-        st.code_left = a[(size_t)row * ap + cl];
-        st.code_right = a[(size_t)row * ap + cr];
-        st.dop_left = a[(size_t)dl * ap + col];
-        st.dop_right = a[(size_t)dr * ap + col];
+        st.code_left = a[(size_t)row * ldr + (size_t)cl * ldc];
+        st.code_right = a[(size_t)row * ldr + (size_t)cr * ldc];
+        st.dop_left = a[(size_t)dl * ldr + (size_t)col * ldc];
+        st.dop_right = a[(size_t)dr * ldr + (size_t)col * ldc];
         stats[p] = st;
     }
+}
+
+// Surface export from a column-major accumulator (column pitch ldc): a
+// strided copy into the Julia column-major (ndop, spc_eff) layout.
+// grid (spc_eff, ceil(ndop / NTHR))
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_export_cm(const float* __restrict__ acc, float* __restrict__ out, int ndop, int ldc)
+{
+    const int col = blockIdx.x, row = blockIdx.y * FMDBZP_NTHR + threadIdx.x;
+    if (row < ndop) out[(size_t)col * ndop + row] = acc[(size_t)col * ldc + row];
 }
 
 // Surface export for store_power_bins: acc[row][col] (row pitch ap) -> Julia column-major
