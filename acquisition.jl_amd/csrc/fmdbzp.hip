@@ -180,6 +180,7 @@ struct fmdbzp_plan {
     int tp = 0, ap = 0;          // tile / accumulator row pitches (multiples of 32 elements)
     bool col_onebuf = false;     // simple column kernel runs its FFT stages in place
     bool col_static = false;     // simple column kernel uses the compile-time ndop = 160 transform
+    bool rot_direct = false;     // rotation kernel writes transform outputs straight to the accumulator
     int corr_variant = 0;        // k_corr_tile: 3 = compile-time F = 2048, 2 = two in-place transforms per
                                  // pass, 1 = one, 0 = ping-pong
     size_t col_lds = 0;
@@ -486,10 +487,15 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // ping-pong columns, stage and raw-power rows (+ one row of pattern signs)
     // (two ping-pong columns when one column exceeds the in-place register bound)
     P->col_onebuf = P->simple_kernel && ndop <= FMDBZP_COL_MAXV * FMDBZP_COL_NTHR && fft_inplace_ok(P->pd);
+    // rotation path with one pattern per rotation slice (no data-bit combos,
+    // one edge offset) and even ndop: transforms write the accumulator directly
+    P->rot_direct = d.path == FMDBZP_PATH_ROTATION && d.num_data_bits == 1 && d.bit_edge_search_steps == 1 &&
+                    ndop % 2 == 0;
     const size_t per_col = P->simple_kernel ? (size_t)P->bstride * (P->col_onebuf ? 1 : 2) * sizeof(float2)
+                           : P->rot_direct  ? (size_t)P->bstride * 3 * sizeof(float2)
                                             : (size_t)P->bstride * 3 * sizeof(float2) +
                                                   (size_t)P->sst * 2 * sizeof(float);
-    const size_t fixed = d.path == FMDBZP_PATH_ROTATION ? (size_t)ndop * sizeof(float) : 0;
+    const size_t fixed = d.path == FMDBZP_PATH_ROTATION && !P->rot_direct ? (size_t)ndop * sizeof(float) : 0;
     // the sign/rotation kernels add static LDS to the dynamic part
     const size_t budget = P->simple_kernel ? kLdsDefault : kLdsDefault - 1024;
     int LB = budget > fixed ? (int)((budget - fixed) / per_col) : 0;
@@ -738,6 +744,7 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
     A.acc_ps = P->acc_ps;
     A.acc_ldr = P->acc_cm ? 1 : P->ap;
     A.acc_ldc = P->acc_cm ? P->apc : 1;
+    A.rot_direct = P->rot_direct ? 1 : 0;
     const int streamed_rot = (nnc == 1 && d.path == FMDBZP_PATH_ROTATION) ? 1 : 0;
     const int nlb = (bs + P->LB - 1) / P->LB;
 

@@ -258,6 +258,7 @@ struct ColumnArgs {
     // row-major (acc_ldr = ap, acc_ldc = 1) or column-major (acc_ldr = 1, acc_ldc = column pitch)
     size_t acc_ps;
     int acc_ldr, acc_ldc;
+    int rot_direct;  // rotation path with one pattern per slice and even ndop: FFT outputs go straight to acc
 };
 
 // Accumulate (or store, at the first step) one power value with the drift
@@ -546,6 +547,39 @@ struct RotIn {
     }
 };
 
+// Rotation input with the pattern sign looked up per row: +-1 of coherent
+// period n / nb of pattern q.
+struct RotPatIn {
+    const float2* x;
+    const float* pq;  // pattern q, one sign per coherent period
+    const int* eff;
+    int bst, ndop, nb;
+    float inv_nb;
+    __device__ __forceinline__ float2 operator()(int t, int n) const
+    {
+        int src = eff[t] + n;
+        if (src >= ndop) src -= ndop;
+        int rem;
+        const float sg = pq[fdivmod(n, nb, inv_nb, &rem)];
+        const float2 u = x[t * bst + src];
+        return make_float2(sg * u.x, sg * u.y);
+    }
+};
+
+// Power (fma form, as the reference's muladd) of raw bin w straight into the
+// accumulator row (w + ndop/2) mod ndop of rotation slice `slice`: the
+// sign-path fold for even ndop, with nothing to max-reduce.
+struct RotAccOut {
+    const ColumnArgs* A;
+    int p, cp0, slice, half;
+    __device__ __forceinline__ void operator()(int t, int w, float2 z) const
+    {
+        int row = w + half;
+        if (row >= A->ndop) row -= A->ndop;
+        acc_cell(*A, p, row, cp0 + t, slice, fmaf(z.x, z.x, __fmul_rn(z.y, z.y)));
+    }
+};
+
 // Secondary-code rotation search (`_rotation_column!`,
 // src/noncoherent_integration.jl:183-297). The reference combines N_coh
 // zero-padded sub-block FFTs with the phasors patterns[p,q]
@@ -576,6 +610,21 @@ k_column_rotation(ColumnArgs A)
     const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
     const int half_block = A.bs >> 1;
     const float inv_nb = 1.0f / (float)A.nb;
+    if (A.rot_direct) {
+        // one pattern per rotation slice (no data-bit combos, one edge offset)
+        for (int c = tid; c < ncol; c += FMDBZP_NTHR) {
+            const int pc = r * A.bs + lag0 + c;
+            const int wrap = ((A.nb - pc / A.bs) % A.nb) * A.bs + (pc % A.bs);
+            seff[c] = ((wrap + half_block) / A.bs) % A.nb;
+        }
+        for (int rot = 0; rot < A.R; rot++) {
+            __syncthreads();  // seff / x ready; the previous transform is done with ya, yb
+            RotPatIn in{x, pat + (size_t)rot * A.ncoh, seff, bst, ndop, A.nb, inv_nb};
+            fft_run<false, true>(ya, yb, A.pd, A.twD, ncol, bst, tid, FMDBZP_NTHR, in,
+                                 RotAccOut{&A, p, r * A.bs + lag0, rot, ndop / 2});
+        }
+        return;
+    }
     for (int rot = 0; rot < A.R; rot++) {
         for (int idx = tid; idx < A.LB * sst; idx += FMDBZP_NTHR) stage[idx] = 0.f;
         for (int e = 0; e < A.E; e++) {
