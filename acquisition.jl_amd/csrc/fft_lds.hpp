@@ -592,3 +592,17 @@ __device__ __forceinline__ void fft3_static(float2* a, const float2* __restrict_
     stage_static_inplace<R1, INV, N, R0, NT, NTHR, BST>(a, tw, tid);
     stage_static<R2, INV, N, R0 * R1, NT, NTHR>(LdsIn<true>{a, BST}, gout, tw, tid);
 }
+
+// Four-stage variant, R0 x R1 x R2 x R3 = N.
+template <bool INV, int N, int NT, int NTHR, int R0, int R1, int R2, int R3, class GIn, class GOut>
+__device__ __forceinline__ void fft4_static(float2* a, const float2* __restrict__ tw, int tid, const GIn& gin,
+                                            const GOut& gout)
+{
+    static_assert(R0 * R1 * R2 * R3 == N, "radix sequence must multiply to N");
+    constexpr int BST = N + (N >> 4) + 1;  // fft_lds_span(N)
+    stage_static<R0, INV, N, 1, NT, NTHR>(gin, LdsOut<true>{a, BST}, tw, tid);
+    __syncthreads();
+    stage_static_inplace<R1, INV, N, R0, NT, NTHR, BST>(a, tw, tid);
+    stage_static_inplace<R2, INV, N, R0 * R1, NT, NTHR, BST>(a, tw, tid);
+    stage_static<R3, INV, N, R0 * R1 * R2, NT, NTHR>(LdsIn<true>{a, BST}, gout, tw, tid);
+}

@@ -181,6 +181,7 @@ struct fmdbzp_plan {
     bool col_onebuf = false;     // simple column kernel runs its FFT stages in place
     bool col_static = false;     // simple column kernel uses the compile-time ndop = 160 transform
     bool rot_direct = false;     // rotation kernel writes transform outputs straight to the accumulator
+    bool rot2000 = false;        // direct rotation at ndop 2000: compile-time kernel, one column per workgroup
     int corr_variant = 0;        // k_corr_tile: 3 = compile-time F = 2048, 2 = two in-place transforms per
                                  // pass, 1 = one, 0 = ping-pong
     size_t col_lds = 0;
@@ -517,11 +518,17 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // every scatter into isolated 4-byte updates; those plans keep the
     // accumulator column-major, with rows contiguous per code column.
     P->acc_cm = !P->simple_kernel && LB < 16;
+    P->rot2000 = P->rot_direct && ndop == 2000;
+    if (P->rot2000) {
+        P->acc_cm = true;
+        LB = P->LB = 1;
+    }
     P->apc = (ndop + 31) & ~31;
     P->acc_ps = P->acc_cm ? (size_t)spc_eff * P->apc : (size_t)ndop * P->ap;
     P->red_cw = 1024;
     P->red_nchunk = (spc_eff + P->red_cw - 1) / P->red_cw;
     P->col_lds = per_col * LB + fixed;
+    if (P->rot2000) P->col_lds = (size_t)(2000 + 2 * fft_lds_span(2000)) * sizeof(float2);
     if (P->col_lds > kLdsMax) return fail(set_err(FMDBZP_EINVAL, "column stage does not fit LDS"));
 
     if ((rc = set_lds_attr((const void*)k_code_fft, P->fft_lds)) ||
@@ -534,7 +541,8 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         (rc = set_lds_attr((const void*)k_column_simple<true>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_simple<false>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_sign, P->col_lds)) ||
-        (rc = set_lds_attr((const void*)k_column_rotation, P->col_lds)))
+        (rc = set_lds_attr((const void*)k_column_rotation, P->col_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_rot2000, P->col_lds)))
         return fail(rc);
 
     // tables
@@ -805,6 +813,8 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                             k_column_simple<false><<<g, FMDBZP_COL_NTHR, P->col_lds, st>>>(A);
                     } else if (d.path == FMDBZP_PATH_SIGN) {
                         k_column_sign<<<g, FMDBZP_NTHR, P->col_lds, st>>>(A);
+                    } else if (P->rot2000) {
+                        k_column_rot2000<<<dim3(bs, nb, cnt), FMDBZP_NTHR, P->col_lds, st>>>(A);
                     } else {
                         k_column_rotation<<<g, FMDBZP_NTHR, P->col_lds, st>>>(A);
                     }

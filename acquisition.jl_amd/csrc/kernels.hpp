@@ -656,6 +656,71 @@ k_column_rotation(ColumnArgs A)
     }
 }
 
+// Direct rotation path at ndop = 2000 (2000 = 16 x 5 x 5 x 5, compile-time):
+// one column per workgroup, two rotation slices per pass.
+struct RotPat2In {
+    const float2* x;
+    const float* pat;  // patterns of this PRN, [Q][ncoh]
+    int q0, nq, ncoh, eff, nb;
+    float inv_nb;
+    __device__ __forceinline__ float2 operator()(int t, int n) const
+    {
+        int src = eff + n;
+        if (src >= 2000) src -= 2000;
+        const int q = t < nq ? q0 + t : q0;
+        int rem;
+        const float sg = pat[q * ncoh + fdivmod(n, nb, inv_nb, &rem)];
+        const float2 u = x[src];
+        return make_float2(sg * u.x, sg * u.y);
+    }
+};
+struct RotAcc2Out {
+    const ColumnArgs* A;
+    int p, cp, q0, nq;
+    __device__ __forceinline__ void operator()(int t, int w, float2 z) const
+    {
+        if (t >= nq) return;
+        int row = w + 1000;
+        if (row >= 2000) row -= 2000;
+        acc_cell(*A, p, row, cp, q0 + t, fmaf(z.x, z.x, __fmul_rn(z.y, z.y)));
+    }
+};
+
+// grid (bs, nb, batch); dynamic LDS: the column (2000 complex) + two padded
+// transform buffers
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_column_rot2000(ColumnArgs A)
+{
+    constexpr int N = 2000;
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x;
+    const int lbk = xcd_logical_block();
+    const int lag = lbk % gridDim.x, r = (lbk / gridDim.x) % gridDim.y, p = lbk / (gridDim.x * gridDim.y);
+    float2* x = lds;
+    float2* a = lds + N;
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * N) * A.tp + lag;
+    for (int k = tid; k < N; k += FMDBZP_NTHR) x[k] = T[(size_t)k * A.tp];
+    const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
+    const int pc = r * A.bs + lag;
+    // per-column alignment to the primary-code wrap (src/noncoherent_integration.jl:212-215)
+    const int wrap = ((A.nb - pc / A.bs) % A.nb) * A.bs + (pc % A.bs);
+    const int eff = ((wrap + (A.bs >> 1)) / A.bs) % A.nb;
+    const float inv_nb = 1.0f / (float)A.nb;
+#pragma unroll 1
+    for (int q0 = 0; q0 < A.R; q0 += 2) {
+        __syncthreads();  // x ready; the previous pass is done with the buffers
+        // An opaque copy of the thread index keeps the transform's index
+        // arithmetic and twiddle loads inside the loop: hoisted out of it,
+        // they would all stay live in registers across passes.
+        int tid_pass;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(tid_pass) : "v"(tid));
+        const int nq = min(2, A.R - q0);
+        fft4_static<false, N, 2, FMDBZP_NTHR, 16, 5, 5, 5>(a, A.twD, tid_pass,
+                                                            RotPat2In{x, pat, q0, nq, A.ncoh, eff, A.nb, inv_nb},
+                                                            RotAcc2Out{&A, p, pc, q0, nq});
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Reductions: peak with the reference's tie-break order, FP64 row sums.
 // ---------------------------------------------------------------------------
