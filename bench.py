@@ -124,6 +124,9 @@ def main():
     ap.add_argument("--literal-edges", action="store_true",
                     help="evaluate every bit-edge offset (the default evaluates one: with one data bit per "
                          "coherent block the offsets are circular rotations of the column)")
+    ap.add_argument("--workload", choices=("c2", "c1", "c4"), default="c2",
+                    help="c2 (default): GPS L1 C/A 16.368 MHz int16 IF, 32 PRNs, N_coh 10, N_nc 8; "
+                         "c1: GPS L1 C/A 5 MHz, 1 ms, 32 PRNs; c4: E1-like 25 MHz, 36 PRNs, N_coh 25, R 25, N_nc 4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-prns", type=int, default=16)
@@ -138,7 +141,8 @@ def main():
         hbm_stream(int(args.hbm_stream * 2**20))
         return
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary, f"E{args.edge_steps}-{'literal' if args.literal_edges else 'one-edge'}")
+        pmc_summary(args.pmc_summary,
+                    f"{args.workload}-E{args.edge_steps}-{'literal' if args.literal_edges else 'one-edge'}")
         return
     if args.copy_calibration > 0:
         copy_calibration(int(args.copy_calibration * 2**20))
@@ -154,12 +158,28 @@ def main():
     from acquisition.signals import l1ca_if_int16_scene
 
     t0 = time.time()
-    sc = l1ca_if_int16_scene(seed=0x5EED0002)
-    log(f"scene: {sc.signal.shape[0]} int16 samples, present {sc.present} ({time.time() - t0:.1f}s)")
-    prns = list(range(1, 33))
-    plan = plan_acquire(sc.system, sc.sampling_freq, prns, num_coherently_integrated_code_periods=10,
-                        bit_edge_search_steps=args.edge_steps, num_noncoherent_accumulations=8,
-                        min_doppler_coverage=7000.0, literal_edges=args.literal_edges)
+    if args.workload == "c2":
+        sc = l1ca_if_int16_scene(seed=0x5EED0002)
+        plan_kw = dict(num_coherently_integrated_code_periods=10, bit_edge_search_steps=args.edge_steps,
+                       num_noncoherent_accumulations=8, min_doppler_coverage=7000.0,
+                       literal_edges=args.literal_edges)
+        workload = ("GPS L1 C/A 16.368 MHz real int16 at IF 4.092 MHz, 80 ms; PRNs 1-32 (8 present); "
+                    f"N_coh 10, bit-edge steps {args.edge_steps}, N_nc 8, +-7 kHz")
+    elif args.workload == "c1":
+        from acquisition.signals import l1ca_5mhz_scene
+        sc = l1ca_5mhz_scene()
+        plan_kw = dict(min_doppler_coverage=7000.0)
+        workload = "GPS L1 C/A 5 MHz complex baseband, 1 ms; PRNs 1-32 (1 present); N_coh 1, N_nc 1, +-7 kHz"
+    else:
+        from acquisition.signals import e1like_scene
+        sc = e1like_scene(n_periods=25, nnc=4, fs=25e6, nprn=36, npresent=6)
+        plan_kw = dict(num_coherently_integrated_code_periods=25, num_noncoherent_accumulations=4,
+                       min_doppler_coverage=7000.0)
+        workload = ("E1-like pilots (4092-chip codes, BOC(1,1), 25-chip secondary) at 25 MHz, 4 x 100 ms; "
+                    "PRNs 1-36 (6 present); N_coh 25, R 25, N_nc 4, +-7 kHz")
+    log(f"scene: {sc.signal.shape[0]} samples, present {sc.present} ({time.time() - t0:.1f}s)")
+    prns = list(sc.prns)
+    plan = plan_acquire(sc.system, sc.sampling_freq, prns, **plan_kw)
     dev = DeviceArray(sc.signal)
     ndop, spc, R = plan.num_doppler_bins, plan.samples_per_code, plan.num_secondary_rotations
     cells = len(prns) * ndop * spc * R
@@ -208,7 +228,8 @@ def main():
     roofline = None
     if bpl:
         ach = bpl / (avg_ms * 1e-3) / 1e9
-        config_key = f"E{plan.bit_edge_search_steps}-{'literal' if args.literal_edges else 'one-edge'}"
+        config_key = (f"{args.workload}-E{plan.bit_edge_search_steps}-"
+                      f"{'literal' if args.literal_edges else 'one-edge'}")
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, config_key),
                     "kernel": dom,
@@ -216,11 +237,11 @@ def main():
     kernel_share = {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload != "c4":
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
-        sub = list(range(args.cpu_prns))
+        sub = list(range(min(args.cpu_prns, len(prns))))
         t0 = time.perf_counter()
         oracle.acquire(plan, sc.signal, sub, if_hz=sc.interm_freq, nthreads=args.cpu_threads)
         cdt = time.perf_counter() - t0
@@ -243,12 +264,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "GPS L1 C/A 16.368 MHz real int16 at IF 4.092 MHz, 80 ms; PRNs 1-32 (8 present); "
-                               f"N_coh 10, bit-edge steps {plan.bit_edge_search_steps}, N_nc 8, +-7 kHz",
+        "config": {"workload": workload,
                    "prns": len(prns), "samples_per_code": spc, "num_doppler_bins": ndop,
-                   "num_blocks": nb, "fft_size": F, "num_coh": 10,
+                   "num_blocks": nb, "fft_size": F, "num_coh": plan.num_coherently_integrated_code_periods,
                    "bit_edge_search_steps": plan.bit_edge_search_steps, "num_noncoherent": nnc,
-                   "precision": "fp32", "signal": "device-resident int16",
+                   "secondary_rotations": R, "precision": "fp32",
+                   "signal": f"device-resident {sc.signal.dtype}",
                    "edge_evaluation": "literal" if args.literal_edges else "one-edge identity"},
         "prn_acqs_per_s": round(len(prns) / dt, 2),
         "host_signal_cells_per_s": round(cells / dt_host, 1),
