@@ -80,6 +80,40 @@ class Request(ctypes.Structure):
     ]
 
 
+class Emitter(ctypes.Structure):
+    _fields_ = [
+        ("chips", ctypes.c_void_p),
+        ("code_length", ctypes.c_int32),
+        ("boc", ctypes.c_int32),
+        ("secondary", ctypes.c_void_p),
+        ("secondary_length", ctypes.c_int32),
+        ("secondary_phase", ctypes.c_int32),
+        ("bits", ctypes.c_void_p),
+        ("num_bits", ctypes.c_int32),
+        ("bit_ms", ctypes.c_int32),
+        ("bit_edge_ms", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("amplitude", ctypes.c_double),
+        ("code_freq_hz", ctypes.c_double),
+        ("code_phase_chips", ctypes.c_double),
+        ("carrier_hz", ctypes.c_double),
+        ("carrier_phase", ctypes.c_double),
+    ]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_samples", ctypes.c_int64),
+        ("sampling_freq_hz", ctypes.c_double),
+        ("complex_samples", ctypes.c_int32),
+        ("out_eltype", ctypes.c_int32),
+        ("noise_scale", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("noise_stream", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 EXPORTED_SYMBOLS = (
     "fmdbzp_plan_create",
     "fmdbzp_acquire",
@@ -89,6 +123,7 @@ EXPORTED_SYMBOLS = (
     "fmdbzp_query",
     "fmdbzp_plan_set_timing",
     "fmdbzp_plan_kernel_times",
+    "fmdbzp_synthesize",
 )
 
 
@@ -116,6 +151,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fmdbzp_plan_set_timing.restype = ctypes.c_int
     lib.fmdbzp_plan_kernel_times.argtypes = [P, P, P, ctypes.c_int32]
     lib.fmdbzp_plan_kernel_times.restype = ctypes.c_int
+    lib.fmdbzp_synthesize.argtypes = [ctypes.POINTER(SceneDesc), P, ctypes.c_int32, P]
+    lib.fmdbzp_synthesize.restype = ctypes.c_int
     return lib
 
 

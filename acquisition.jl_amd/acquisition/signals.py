@@ -93,21 +93,104 @@ def l1ca_5mhz_scene(seed: int = 0x5EED0001, cn0: float = 45.0) -> Scene:
                  dict(prn=prn, code_phase=cp, doppler=fd, cn0=cn0))
 
 
+@dataclass
+class Emitter:
+    """One transmitter of a synthetic scene. At sample n (time n / fs), with
+    pos = code_phase + (n code_freq) / fs and c = floor(pos), it adds
+    amplitude x chips[c mod L] x (-1 if boc and pos - c >= 1/2)
+    x secondary[(floor(c / L) + secondary_phase) mod len(secondary)]
+    x bits[min((floor(1000 n / fs) + bit_edge_ms) // bit_ms, len(bits) - 1)]
+    x cos(ang) (+ i sin(ang) for complex scenes), ang = (2 pi carrier_hz) n / fs + carrier_phase,
+    the code (with code Doppler) x carrier model of src/generate_test_signal.jl:48-76."""
+    chips: np.ndarray
+    amplitude: float
+    code_freq: float
+    code_phase: float
+    carrier_hz: float
+    carrier_phase: float
+    boc: bool = False
+    secondary: np.ndarray = None
+    secondary_phase: int = 0
+    bits: np.ndarray = None
+    bit_ms: int = 20
+    bit_edge_ms: int = 0
+
+
+def render_host(n: int, fs: float, complex_samples: bool, out_dtype, noise_scale: float, seed: int,
+                noise_stream: int, emitters) -> np.ndarray:
+    """The scene on the host, in FP64 with the operation order of the device
+    generator (`acquisition.device.synthesize`): Philox Box-Muller noise
+    x `noise_scale` (one counter block gives the two components of a complex
+    sample, or two consecutive real samples), plus the emitters in order,
+    rounded once to `out_dtype` (int16: half to even, saturated)."""
+    t = np.arange(n, dtype=np.float64)
+    if complex_samples:
+        g = PhiloxStream(seed, noise_stream).normal(2 * n)
+        re = g[0::2] * noise_scale
+        im = g[1::2] * noise_scale
+    else:
+        re = PhiloxStream(seed, noise_stream).normal(n) * noise_scale
+        im = None
+    for em in emitters:
+        chips = np.asarray(em.chips, dtype=np.int64)
+        L = len(chips)
+        pos = em.code_phase + (t * em.code_freq) / fs
+        fpos = np.floor(pos)
+        c = fpos.astype(np.int64)
+        chip = chips[np.mod(c, L)]
+        if em.boc:
+            chip = np.where(pos - fpos < 0.5, chip, -chip)
+        if em.secondary is not None:
+            sec = np.asarray(em.secondary, dtype=np.int64)
+            chip = chip * sec[np.mod(np.floor_divide(c, L) + em.secondary_phase, len(sec))]
+        if em.bits is not None:
+            bits = np.asarray(em.bits, dtype=np.int64)
+            ms = np.floor((t * 1000.0) / fs).astype(np.int64)
+            chip = chip * bits[np.minimum((ms + em.bit_edge_ms) // em.bit_ms, len(bits) - 1)]
+        ang = ((2 * math.pi * em.carrier_hz) * t) / fs + em.carrier_phase
+        a = em.amplitude * chip.astype(np.float64)
+        re += a * np.cos(ang)
+        if complex_samples:
+            im += a * np.sin(ang)
+    dt = np.dtype(out_dtype)
+    if complex_samples:
+        if dt != np.complex64:
+            raise ValueError("complex scenes are rendered as complex64")
+        out = np.empty(n, dtype=np.complex64)
+        out.real = re.astype(np.float32)
+        out.imag = im.astype(np.float32)
+        return out
+    if dt == np.int16:
+        return np.clip(np.rint(re), -32768, 32767).astype(np.int16)
+    if dt == np.float32:
+        return re.astype(np.float32)
+    raise ValueError(f"unsupported output dtype {dt}")
+
+
+def _render(device: bool, *args):
+    if device:
+        from .device import synthesize
+
+        return synthesize(*args)
+    return render_host(*args)
+
+
 def l1ca_if_int16_scene(seed: int = 0x5EED0002, n_ms: int = 80, fs: float = 16.368e6, if_hz: float = 4.092e6,
-                        npresent: int = 8, sigma: float = 1000.0, cn0_range=(30.0, 40.0)) -> Scene:
+                        npresent: int = 8, sigma: float = 1000.0, cn0_range=(30.0, 40.0),
+                        device: bool = False) -> Scene:
     """Real int16 IF samples: `npresent` of PRNs 1..32 at CN0 U[cn0_range],
     Doppler U[-5,5] kHz, code phase U[0,1023), +-1 data at 50 bps with a
     random bit-edge offset in whole ms, Gaussian noise sigma LSB.
-    C/N0 = A^2 fs / (4 sigma^2) for a real carrier A cos(.)."""
+    C/N0 = A^2 fs / (4 sigma^2) for a real carrier A cos(.).
+    `device=True` generates the samples in device memory (a `DeviceArray`)."""
     sysm = GPSL1CA()
     n = int(round(fs * n_ms / 1000.0))
     st = PhiloxStream(seed, 0)
     perm = np.argsort(st.uniform(32))
     present = sorted(int(p) + 1 for p in perm[:npresent])
-    t = np.arange(n, dtype=np.float64)
-    x = PhiloxStream(seed, 1).normal(n) * sigma
+    emitters = []
     truth = {}
-    for i, prn in enumerate(present):
+    for prn in present:
         u = PhiloxStream(seed, 100 + prn).uniform(5)
         cn0 = cn0_range[0] + (cn0_range[1] - cn0_range[0]) * u[0]
         fd = -5000.0 + 10000.0 * u[1]
@@ -116,31 +199,29 @@ def l1ca_if_int16_scene(seed: int = 0x5EED0002, n_ms: int = 80, fs: float = 16.3
         edge = int(u[4] * 20)
         amp = math.sqrt(4 * sigma ** 2 * 10 ** (cn0 / 10) / fs)
         code_freq = sysm.code_frequency + fd * sysm.get_code_center_frequency_ratio()
-        code = sysm.gen_code(n, prn, fs, code_freq, cp).astype(np.float64)
-        nbits = n_ms // 20 + 2
-        bits = PhiloxStream(seed, 200 + prn).signs(nbits).astype(np.float64)
-        ms = np.floor(t * 1000.0 / fs).astype(np.int64)
-        data = bits[(ms + edge) // 20]
-        x += amp * data * code * np.cos(2 * math.pi * (if_hz + fd) * t / fs + ph)
+        bits = PhiloxStream(seed, 200 + prn).signs(n_ms // 20 + 2)
+        emitters.append(Emitter(sysm.primary_chips(prn), amp, code_freq, cp, if_hz + fd, ph,
+                                bits=bits, bit_ms=20, bit_edge_ms=edge))
         truth[prn] = dict(cn0=cn0, doppler=fd, code_phase=cp, edge_ms=edge)
-    sig = np.clip(np.rint(x), -32768, 32767).astype(np.int16)
+    sig = _render(device, n, fs, False, np.int16, sigma, seed, 1, emitters)
     return Scene(sig, fs, if_hz, sysm, list(range(1, 33)), present, truth)
 
 
 def e1like_scene(seed: int = 0x5EED0004, n_periods: int = 100, nnc: int = 4, fs: float = 25e6,
-                 nprn: int = 36, npresent: int = 6, cn0_range=(28.0, 38.0), system=None) -> Scene:
+                 nprn: int = 36, npresent: int = 6, cn0_range=(28.0, 38.0), system=None,
+                 device: bool = False) -> Scene:
     """E1-like pilots: complex baseband at fs, `nnc` segments of `n_periods`
     4 ms primary periods; `npresent` of `nprn` PRNs with secondary phase
-    U{0..24}, Doppler U[-5,5] kHz, CN0 U[cn0_range]; unit-variance noise."""
+    U{0..24}, Doppler U[-5,5] kHz, CN0 U[cn0_range]; unit-variance noise.
+    `device=True` generates the samples in device memory (a `DeviceArray`)."""
     sysm = system or E1Like()
     spc = math.ceil(sysm.code_length / sysm.code_frequency * fs)
     n = spc * n_periods * nnc
     st = PhiloxStream(seed, 0)
     perm = np.argsort(st.uniform(nprn))
     present = sorted(int(p) + 1 for p in perm[:npresent])
-    t = np.arange(n, dtype=np.float64)
-    x = _noise_c(PhiloxStream(seed, 1), n) / math.sqrt(2.0)
     L = sysm.get_secondary_code_length()
+    emitters = []
     truth = {}
     for prn in present:
         u = PhiloxStream(seed, 100 + prn).uniform(5)
@@ -151,14 +232,12 @@ def e1like_scene(seed: int = 0x5EED0004, n_periods: int = 100, nnc: int = 4, fs:
         sec0 = int(u[4] * L)
         amp = math.sqrt(10 ** (cn0 / 10) / fs)
         code_freq = sysm.code_frequency + fd * sysm.get_code_center_frequency_ratio()
-        pos = cp + (t * code_freq) / fs
-        code = sysm.gen_code(n, prn, fs, code_freq, cp).astype(np.float64)
-        per = np.floor(pos / sysm.code_length).astype(np.int64)
-        sec = sysm.secondary_code(prn).astype(np.float64)
-        code = code * sec[(per + sec0) % L]
-        x += amp * code * np.exp(1j * (2 * math.pi * fd * t / fs + ph))
+        emitters.append(Emitter(sysm.primary_chips(prn), amp, code_freq, cp, fd, ph, boc=sysm.boc,
+                                secondary=sysm.secondary_code(prn), secondary_phase=sec0))
         truth[prn] = dict(cn0=cn0, doppler=fd, code_phase=cp, secondary_phase=sec0)
-    return Scene(x.astype(np.complex64), fs, 0.0, sysm, list(range(1, nprn + 1)), present, truth)
+    # unit total variance: each component N(0, 1/2)
+    sig = _render(device, n, fs, True, np.complex64, 1.0 / math.sqrt(2.0), seed, 1, emitters)
+    return Scene(sig, fs, 0.0, sysm, list(range(1, nprn + 1)), present, truth)
 
 
 def l1ca_quantized_scene(kind: str = "ci8", seed: int = 0x5EED0003, n_ms: int = 80, fs: float = 16.368e6,

@@ -24,6 +24,7 @@
 
 #include "../../include/fmdbzp.h"
 #include "kernels.hpp"
+#include "synth.hpp"
 
 namespace {
 
@@ -964,3 +965,87 @@ int fmdbzp_acquire_batch(const fmdbzp_request* reqs, int32_t nreqs)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Seeded synthetic signals on the device (fmdbzp.h).
+// ---------------------------------------------------------------------------
+extern "C" int fmdbzp_synthesize(const fmdbzp_scene* scene, const fmdbzp_emitter* emitters, int32_t n_emitters,
+                                 void* out)
+{
+    if (!scene || !out || n_emitters < 0 || (n_emitters > 0 && !emitters))
+        return set_err(FMDBZP_EINVAL, "null scene, emitters or output");
+    const fmdbzp_scene& sc = *scene;
+    if (sc.n_samples <= 0 || !(sc.sampling_freq_hz > 0.0))
+        return set_err(FMDBZP_EINVAL, "n_samples and sampling_freq_hz must be positive");
+    const bool cplx = sc.complex_samples != 0;
+    if (cplx ? sc.out_eltype != FMDBZP_CF32 : (sc.out_eltype != FMDBZP_I16 && sc.out_eltype != FMDBZP_F32))
+        return set_err(FMDBZP_EINVAL, "out_eltype must be CF32 for complex scenes, I16 or F32 for real ones");
+    std::vector<int8_t> table;
+    std::vector<SynthEmitter> em((size_t)n_emitters);
+    auto append = [&](const int8_t* v, int64_t n, int64_t* off) -> bool {
+        *off = (int64_t)table.size();
+        for (int64_t i = 0; i < n; i++) {
+            if (v[i] != 1 && v[i] != -1) return false;
+            table.push_back(v[i]);
+        }
+        return true;
+    };
+    for (int e = 0; e < n_emitters; e++) {
+        const fmdbzp_emitter& E = emitters[e];
+        SynthEmitter& D = em[(size_t)e];
+        if (!E.chips || E.code_length <= 0 || !(E.code_freq_hz >= 0.0))
+            return set_err(FMDBZP_EINVAL, "emitter %d: chips and a positive code_length are required", e);
+        if ((E.secondary && E.secondary_length <= 0) ||
+            (E.bits && (E.num_bits <= 0 || E.bit_ms <= 0 || E.bit_edge_ms < 0)))
+            return set_err(FMDBZP_EINVAL, "emitter %d: secondary / bit table sizes or a negative bit_edge_ms", e);
+        memset(&D, 0, sizeof(D));
+        if (!append(E.chips, E.code_length, &D.chips_off) ||
+            (E.secondary && !append(E.secondary, E.secondary_length, &D.sec_off)) ||
+            (E.bits && !append(E.bits, E.num_bits, &D.bits_off)))
+            return set_err(FMDBZP_EINVAL, "emitter %d: chip, secondary and bit values must be +-1", e);
+        D.code_length = E.code_length;
+        D.boc = E.boc != 0;
+        D.sec_len = E.secondary ? E.secondary_length : 0;
+        D.sec_phase = E.secondary_phase;
+        D.num_bits = E.bits ? E.num_bits : 0;
+        D.bit_ms = E.bits ? E.bit_ms : 1;
+        D.bit_edge_ms = E.bit_edge_ms;
+        D.amplitude = E.amplitude;
+        D.code_freq = E.code_freq_hz;
+        D.code_phase = E.code_phase_chips;
+        D.carrier_hz = E.carrier_hz;
+        D.carrier_phase = E.carrier_phase;
+    }
+    if (table.empty()) table.push_back(1);
+    int8_t* d_table = nullptr;
+    SynthEmitter* d_em = nullptr;
+    hipStream_t st = nullptr;
+    auto cleanup = [&]() {
+        if (st) hipStreamDestroy(st);
+        hipFree(d_table);
+        hipFree(d_em);
+    };
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_table, table.size());
+    if (e == hipSuccess && n_emitters > 0) e = hipMalloc((void**)&d_em, sizeof(SynthEmitter) * (size_t)n_emitters);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_table, table.data(), table.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && n_emitters > 0)
+        e = hipMemcpyAsync(d_em, em.data(), sizeof(SynthEmitter) * (size_t)n_emitters, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const int64_t nthreads = cplx ? sc.n_samples : (sc.n_samples + 1) / 2;
+        const int64_t nblk = (nthreads + 255) / 256;
+        if (nblk > 0x7FFFFFFF) {
+            cleanup();
+            return set_err(FMDBZP_EINVAL, "n_samples too large");
+        }
+        k_synthesize<<<(unsigned)nblk, 256, 0, st>>>(d_em, n_emitters, d_table, sc.n_samples, sc.sampling_freq_hz,
+                                                     cplx ? 1 : 0, sc.out_eltype == FMDBZP_I16 ? 1 : 0,
+                                                     sc.noise_scale, (uint32_t)(sc.seed & 0xFFFFFFFFu),
+                                                     (uint32_t)(sc.seed >> 32), sc.noise_stream, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    cleanup();
+    if (e != hipSuccess) return set_err(FMDBZP_EHIP, "fmdbzp_synthesize: %s", hipGetErrorString(e));
+    return FMDBZP_OK;
+}

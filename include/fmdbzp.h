@@ -242,6 +242,65 @@ enum fmdbzp_kernel_class {
 int fmdbzp_plan_set_timing(fmdbzp_plan* plan, int32_t enable);
 int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launches, int32_t n);
 
+/* --------------------------------------------------------------------------
+ * Seeded synthetic signals generated on the device: the analogue of
+ * `generate_test_signal` (src/generate_test_signal.jl:32-82) for benchmark
+ * scenes of many emitters. Samples are noise plus the emitters' terms, added
+ * in emitter order, each evaluated in FP64:
+ *   pos   = code_phase_chips + (n * code_freq_hz) / sampling_freq_hz,  c = floor(pos)
+ *   chip  = chips[c mod code_length]
+ *           (* -1 where pos - c >= 1/2, when boc != 0)
+ *           (* secondary[(floor(c / code_length) + secondary_phase) mod secondary_length])
+ *           (* bits[min((floor(n * 1000 / sampling_freq_hz) + bit_edge_ms) / bit_ms,
+ *                       num_bits - 1)], bit_edge_ms >= 0)
+ *   angle = (2 pi carrier_hz) * n / sampling_freq_hz + carrier_phase
+ *   real scenes:    x += (amplitude * chip) * cos(angle)
+ *   complex scenes: x += (amplitude * chip) * (cos(angle) + i sin(angle))
+ * Noise is Philox4x32-10 (key = seed, counter = (block, noise_stream, 0)) with
+ * Box-Muller in FP64, one counter block per complex sample or per pair of
+ * real samples, times noise_scale. Output: FMDBZP_CF32 (complex scenes),
+ * FMDBZP_I16 (real; round half to even, saturated) or FMDBZP_F32 (real).
+ * The same samples, up to the last bit of log / sqrt / cos / sin, come from
+ * the host generator in acquisition.signals (render_host).
+ * fmdbzp_synthesize rejects (FMDBZP_EINVAL, before any device work) null pointers, non-positive
+ * sizes, an output type that does not fit the scene, and tables holding
+ * values other than +-1.
+ * -------------------------------------------------------------------------- */
+typedef struct fmdbzp_emitter {
+    const int8_t* chips;      /* primary code, +-1, code_length entries (host) */
+    int32_t code_length;
+    int32_t boc;              /* 1: BOC(1,1) square subcarrier */
+    const int8_t* secondary;  /* +-1 per primary period, or NULL */
+    int32_t secondary_length;
+    int32_t secondary_phase;
+    const int8_t* bits;       /* +-1 data bits, or NULL */
+    int32_t num_bits;
+    int32_t bit_ms;           /* data bit length in whole ms */
+    int32_t bit_edge_ms;
+    int32_t reserved;
+    double amplitude;
+    double code_freq_hz;      /* including code Doppler */
+    double code_phase_chips;
+    double carrier_hz;        /* IF + Doppler */
+    double carrier_phase;
+} fmdbzp_emitter;
+
+typedef struct fmdbzp_scene {
+    int64_t n_samples;
+    double sampling_freq_hz;
+    int32_t complex_samples;  /* 0: real IF samples, 1: complex baseband */
+    int32_t out_eltype;       /* FMDBZP_CF32 (complex); FMDBZP_I16 or FMDBZP_F32 (real) */
+    double noise_scale;
+    uint64_t seed;
+    uint32_t noise_stream;
+    int32_t reserved;
+} fmdbzp_scene;
+
+/* Write scene->n_samples samples into `out` (device memory of the current
+ * device, sized for out_eltype). Synchronous. */
+int fmdbzp_synthesize(const fmdbzp_scene* scene, const fmdbzp_emitter* emitters, int32_t n_emitters,
+                      void* out);
+
 #ifdef __cplusplus
 }
 #endif

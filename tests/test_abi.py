@@ -38,6 +38,9 @@ def test_struct_layouts_match_header(tmp_path):
 int main(void) {
   printf("%zu %zu %zu %zu\\n", sizeof(fmdbzp_plan_desc), offsetof(fmdbzp_plan_desc, sampling_freq_hz),
          sizeof(fmdbzp_prn_stats), offsetof(fmdbzp_prn_stats, noise_cells));
+  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(fmdbzp_emitter), offsetof(fmdbzp_emitter, bits),
+         offsetof(fmdbzp_emitter, carrier_phase), sizeof(fmdbzp_scene), offsetof(fmdbzp_scene, seed),
+         offsetof(fmdbzp_scene, noise_stream));
   return 0;
 }
 """)
@@ -48,6 +51,33 @@ int main(void) {
     assert int(out[1]) == _native.PlanDesc.sampling_freq_hz.offset
     assert int(out[2]) == ctypes.sizeof(_native.PrnStats)
     assert int(out[3]) == _native.PrnStats.noise_cells.offset
+    E, S = _native.Emitter, _native.SceneDesc
+    assert [int(v) for v in out[4:]] == [ctypes.sizeof(E), E.bits.offset, E.carrier_phase.offset,
+                                         ctypes.sizeof(S), S.seed.offset, S.noise_stream.offset]
+
+
+def test_synthesize_rejects_bad_scenes():
+    """Validation happens before any device work, so it runs without a GPU."""
+    lib = _native.load_library()
+    chips = np.ones(8, dtype=np.int8)
+    em = _native.Emitter(chips=chips.ctypes.data, code_length=8, amplitude=1.0, code_freq_hz=1e3)
+    sc = _native.SceneDesc(n_samples=16, sampling_freq_hz=1e4, complex_samples=0, out_eltype=_native.ELTYPE["i16"])
+    out = ctypes.c_void_p(1)
+    assert lib.fmdbzp_synthesize(None, None, 0, out) == -1
+    sc.out_eltype = _native.ELTYPE["cf32"]
+    assert lib.fmdbzp_synthesize(ctypes.byref(sc), ctypes.byref(em), 1, out) == -1
+    assert b"out_eltype" in lib.fmdbzp_last_error()
+    sc.out_eltype = _native.ELTYPE["i16"]
+    chips[3] = 0
+    assert lib.fmdbzp_synthesize(ctypes.byref(sc), ctypes.byref(em), 1, out) == -1
+    assert b"+-1" in lib.fmdbzp_last_error()
+    chips[3] = 1
+    bits = np.ones(4, dtype=np.int8)
+    em.bits, em.num_bits, em.bit_ms, em.bit_edge_ms = bits.ctypes.data, 4, 20, -1
+    assert lib.fmdbzp_synthesize(ctypes.byref(sc), ctypes.byref(em), 1, out) == -1
+    assert b"bit_edge_ms" in lib.fmdbzp_last_error()
+    sc.n_samples = 0
+    assert lib.fmdbzp_synthesize(ctypes.byref(sc), ctypes.byref(em), 1, out) == -1
 
 
 def test_plan_create_rejects_bad_descriptor():
