@@ -284,3 +284,101 @@ def l1ca_quantized_scene(kind: str = "ci8", seed: int = 0x5EED0003, n_ms: int = 
     else:
         raise ValueError(kind)
     return Scene(sig, fs, if_hz, sysm, list(range(1, 33)), present, truth)
+
+
+@dataclass
+class BatchRequest:
+    """One request of the batched mix: geometry, searched PRNs, the present
+    emitters and the noise stream of its samples (complex baseband, unit
+    total noise variance)."""
+    kind: str
+    system: GNSSSystem
+    sampling_freq: float
+    num_coh: int
+    num_nc: int
+    coverage: float
+    prns: List[int]
+    present: List[int]
+    n_samples: int
+    emitters: list
+    seed: int
+    truth: dict = field(default_factory=dict)
+
+    def plan_key(self):
+        return (self.kind, self.sampling_freq, self.num_coh, self.num_nc, self.coverage)
+
+    def signal(self, device: bool = False):
+        return _render(device, self.n_samples, self.sampling_freq, True, np.complex64, 1.0 / math.sqrt(2.0),
+                       self.seed, 1, self.emitters)
+
+
+def mixed_batch_requests(seed: int = 0x5EED0005, n: int = 256, cn0_range=(30.0, 45.0),
+                         coverages=(5000.0, 10000.0, 25000.0, 50000.0), e1_max_coverage: float = 7000.0,
+                         e1_max_prns: int = 4, e1_max_nc: int = 2) -> List[BatchRequest]:
+    """`n` acquisition requests drawn by seed from three types:
+      a  GPS L1 C/A, fs 5 or 16.368 MHz, N_coh 1..10, 1..32 PRNs, 50 bps data;
+      b  L5-like (10230-chip codes at 10.23 Mchip/s, 10-chip secondary code),
+         fs 30.69 MHz, N_coh 1 or 10, 1..36 PRNs;
+      c  E1-like (4092-chip BOC(1,1) codes, 25-chip secondary code), fs 25 MHz,
+         N_coh 25, at most `e1_max_prns` PRNs, N_nc <= `e1_max_nc` and a
+         Doppler coverage of at most `e1_max_coverage`.
+    Types a and b take N_nc 1..8 and a Doppler coverage from `coverages`.
+    About a quarter of each request's PRNs are present (at least one), at
+    CN0 U[cn0_range], Doppler U[-0.9, 0.9] x coverage, uniform code and
+    carrier phase, uniform secondary phase."""
+    from .systems import L5Like
+
+    systems = {"a": GPSL1CA(), "b": L5Like(), "c": E1Like()}
+    out = []
+    for i in range(n):
+        u = PhiloxStream(seed, 1000 + i).uniform(8)
+        kind = "abc"[min(int(u[0] * 3), 2)]
+        sysm = systems[kind]
+        cov = coverages[min(int(u[1] * len(coverages)), len(coverages) - 1)]
+        if kind == "a":
+            fs = 5e6 if u[2] < 0.5 else 16.368e6
+            ncoh = 1 + int(u[3] * 10)
+            nprn = 1 + int(u[4] * 32)
+            nnc = 1 + int(u[5] * 8)
+            pool = 32
+        elif kind == "b":
+            fs = 30.69e6
+            ncoh = 1 if u[3] < 0.5 else 10
+            nprn = 1 + int(u[4] * 36)
+            nnc = 1 + int(u[5] * 8)
+            pool = 36
+        else:
+            fs = 25e6
+            ncoh = 25
+            nprn = 1 + int(u[4] * e1_max_prns)
+            nnc = 1 + int(u[5] * e1_max_nc)
+            cov = min(cov, e1_max_coverage)
+            pool = 36
+        rseed = seed + 0x10000 * (i + 1)
+        perm = np.argsort(PhiloxStream(rseed, 0).uniform(pool))
+        prns = sorted(int(p) + 1 for p in perm[:nprn])
+        npresent = max(1, round(nprn / 4))
+        present = prns[:npresent] if npresent == nprn else \
+            sorted(prns[int(j)] for j in np.argsort(PhiloxStream(rseed, 2).uniform(nprn))[:npresent])
+        spc = math.ceil(sysm.code_length / sysm.code_frequency * fs)
+        n_samples = spc * ncoh * nnc
+        emitters, truth = [], {}
+        sec_len = sysm.get_secondary_code_length()
+        for prn in present:
+            v = PhiloxStream(rseed, 100 + prn).uniform(6)
+            cn0 = cn0_range[0] + (cn0_range[1] - cn0_range[0]) * v[0]
+            fd = 0.9 * cov * (2.0 * v[1] - 1.0)
+            cp = sysm.code_length * v[2]
+            ph = 2 * math.pi * v[3]
+            amp = math.sqrt(10 ** (cn0 / 10) / fs)
+            code_freq = sysm.code_frequency + fd * sysm.get_code_center_frequency_ratio()
+            kw = {}
+            if kind == "a":
+                kw = dict(bits=PhiloxStream(rseed, 200 + prn).signs(n_samples * 50 // int(fs) + 3), bit_ms=20,
+                          bit_edge_ms=int(v[4] * 20))
+            elif sec_len > 1:
+                kw = dict(secondary=sysm.secondary_code(prn), secondary_phase=int(v[5] * sec_len))
+            emitters.append(Emitter(sysm.primary_chips(prn), amp, code_freq, cp, fd, ph, boc=sysm.boc, **kw))
+            truth[prn] = dict(cn0=cn0, doppler=fd, code_phase=cp)
+        out.append(BatchRequest(kind, sysm, fs, ncoh, nnc, cov, prns, present, n_samples, emitters, rseed, truth))
+    return out
