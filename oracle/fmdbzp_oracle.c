@@ -582,12 +582,57 @@ static void o_acquire_prn_streamed(const oplan* P, int prn, const cpx* cache, re
     out->col = pcol;
 }
 
+static void scratch_alloc(oscratch* S, const oplan* P)
+{
+    const fmdbzp_plan_desc* d = &P->d;
+    int ndop = d->num_doppler_bins, F = d->fft_size;
+    int maxn = F > ndop ? F : ndop;
+    int nsub = d->num_data_bits > d->num_coh ? d->num_data_bits : d->num_coh;
+    S->work = (cpx*)malloc(sizeof(cpx) * (size_t)4 * maxn);
+    S->tile = (cpx*)malloc(sizeof(cpx) * (size_t)ndop * d->block_size);
+    S->colbuf = (cpx*)malloc(sizeof(cpx) * (size_t)ndop);
+    S->sub = (cpx*)malloc(sizeof(cpx) * (size_t)ndop * nsub);
+    S->cre = (real*)malloc(sizeof(real) * (size_t)ndop);
+    S->cim = (real*)malloc(sizeof(real) * (size_t)ndop);
+    S->stage = (real*)malloc(sizeof(real) * (size_t)ndop * d->num_secondary_rotations);
+}
+
+static void scratch_free(oscratch* S)
+{
+    free(S->work); free(S->tile); free(S->colbuf); free(S->sub); free(S->cre); free(S->cim); free(S->stage);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Multistep path (N_nc > 1): `_accumulate_prn_step_tiled!`                  */
 /* src/noncoherent_integration.jl:760-812                                    */
 /* ------------------------------------------------------------------------ */
+static void o_accumulate_block(const oplan* P, int prn, const cpx* cache, real* acc, int step, int r,
+                               oscratch* S);
+
+/* Column blocks r of one step write disjoint accumulator cells (column
+ * r*bs + lag, moved by a per-row drift that is a bijection of columns), and
+ * each cell gets one addition per step, so the blocks may run on `inner`
+ * threads in any order with the same result. */
 static void o_accumulate_prn_step(const oplan* P, int prn, const cpx* cache, real* acc, int step,
-                                  oscratch* S)
+                                  oscratch* S, int inner)
+{
+    int nb = P->d.num_blocks;
+    if (inner <= 1) {
+        for (int r = 0; r < nb; r++) o_accumulate_block(P, prn, cache, acc, step, r, S);
+        return;
+    }
+#pragma omp parallel num_threads(inner)
+    {
+        oscratch T;
+        scratch_alloc(&T, P);
+#pragma omp for schedule(dynamic, 1)
+        for (int r = 0; r < nb; r++) o_accumulate_block(P, prn, cache, acc, step, r, &T);
+        scratch_free(&T);
+    }
+}
+
+static void o_accumulate_block(const oplan* P, int prn, const cpx* cache, real* acc, int step, int r,
+                               oscratch* S)
 {
     const fmdbzp_plan_desc* d = &P->d;
     int ndop = d->num_doppler_bins, bs = d->block_size, nb = d->num_blocks, spc = d->samples_per_code;
@@ -595,7 +640,7 @@ static void o_accumulate_prn_step(const oplan* P, int prn, const cpx* cache, rea
     const int32_t* shifts = P->shifts + (size_t)step * ndop;
     int has_drift = 0;
     for (int i = 0; i < ndop; i++) if (shifts[i] != 0) has_drift = 1;
-    for (int r = 0; r < nb; r++) {
+    {
         o_build_tile(S->tile, cache, prnconj, r, nb, bs, d->num_coh, &P->PF, S->work);
         int off = r * bs;
         if (d->path == FMDBZP_PATH_SIMPLE) {
@@ -837,7 +882,7 @@ int oracle_acquire(const fmdbzp_plan_desc* desc, int nprn, const int8_t* codes,
     const fmdbzp_plan_desc* d = &P.d;
     int spc = d->samples_per_code, nb = d->num_blocks, bs = d->block_size, F = d->fft_size;
     int ncoh = d->num_coh, ndop = d->num_doppler_bins, nnc = d->num_noncoherent;
-    int spc_eff = d->samples_per_code_eff, Q = d->num_patterns, R = d->num_secondary_rotations;
+    int spc_eff = d->samples_per_code_eff, Q = d->num_patterns;
     int64_t seglen = (int64_t)ncoh * spc;
     if (nsamp / seglen < nnc) return -1;
     P.nprn = nprn;
@@ -884,18 +929,15 @@ int oracle_acquire(const fmdbzp_plan_desc* desc, int nprn, const int8_t* codes,
 
     int nthr = opts->nthreads > 0 ? opts->nthreads : 1;
     int status = 0;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthr)
+    /* fewer multistep PRNs than threads: PRNs one after another, each step's
+     * column blocks spread over the threads */
+    int inner = (nnc > 1 && nreq < nthr) ? nthr : 1;
+    int outer = inner > 1 ? 1 : nthr;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(outer)
     for (int ri = 0; ri < nreq; ri++) {
         int prn = prn_index[ri];
         oscratch S;
-        S.work = (cpx*)malloc(sizeof(cpx) * worksz);
-        S.tile = (cpx*)malloc(sizeof(cpx) * (size_t)ndop * bs);
-        S.colbuf = (cpx*)malloc(sizeof(cpx) * (size_t)ndop);
-        int nsub = d->num_data_bits > ncoh ? d->num_data_bits : ncoh;
-        S.sub = (cpx*)malloc(sizeof(cpx) * (size_t)ndop * nsub);
-        S.cre = (real*)malloc(sizeof(real) * (size_t)ndop);
-        S.cim = (real*)malloc(sizeof(real) * (size_t)ndop);
-        S.stage = (real*)malloc(sizeof(real) * (size_t)ndop * R);
+        scratch_alloc(&S, &P);
         real* surface = (real*)calloc((size_t)ndop * spc_eff, sizeof(real));
         real* col_sums = (real*)malloc(sizeof(real) * (size_t)spc_eff);
         real* row_sums = (real*)malloc(sizeof(real) * (size_t)ndop);
@@ -911,7 +953,7 @@ int oracle_acquire(const fmdbzp_plan_desc* desc, int nprn, const int8_t* codes,
             }
         } else {
             for (int s = 0; s < nnc; s++)
-                o_accumulate_prn_step(&P, prn, cache + (size_t)s * ncoh * nb * F, surface, s, &S);
+                o_accumulate_prn_step(&P, prn, cache + (size_t)s * ncoh * nb * F, surface, s, &S, inner);
             o_findmax_and_colsums(surface, ndop, spc_eff, col_sums, &pk);
             if (d->noise_estimator == FMDBZP_NOISE_GLOBAL_MEAN) {
                 noise = o_global_mean_from_colsums(col_sums, spc_eff, ndop, pk.col, d->samples_per_chip);
@@ -925,8 +967,8 @@ int oracle_acquire(const fmdbzp_plan_desc* desc, int nprn, const int8_t* codes,
             double* dst = surfaces + (size_t)ri * ndop * spc_eff;
             for (size_t i = 0; i < (size_t)ndop * spc_eff; i++) dst[i] = (double)surface[i];
         }
-        free(S.work); free(S.tile); free(S.colbuf); free(S.sub); free(S.cre); free(S.cim);
-        free(S.stage); free(surface); free(col_sums); free(row_sums);
+        scratch_free(&S);
+        free(surface); free(col_sums); free(row_sums);
     }
 
     free(cache);

@@ -299,6 +299,25 @@ def test_e1like_rotation_boc_reduced():
     assert res[0].secondary_code_phase == sc.truth[p0]["secondary_phase"]
 
 
+def test_rotation_ndop2000_two_steps():
+    """E1-like pilots at 1 MHz, N_coh 25, +-9 kHz: nb 80, ndop 2000, the
+    geometry of the 25 MHz E1-like search (whose compile-time 2000-point
+    rotation kernel this exercises) on 4000-sample codes; two noncoherent
+    steps, so the second accumulates onto the first."""
+    from acquisition import E1Like
+    from acquisition.signals import e1like_scene
+
+    sysm = E1Like()
+    sc = e1like_scene(n_periods=25, nnc=2, fs=1e6, nprn=4, npresent=2, cn0_range=(46.0, 48.0), system=sysm)
+    p0 = sc.present[0]
+    prns = [p0, 1 if p0 != 1 else 4]
+    plan, res, ores = run_case(sysm, sc.sampling_freq, prns, sc.signal, interp=True, label="rot2000",
+                               min_doppler_coverage=9000.0, num_coherently_integrated_code_periods=25,
+                               num_noncoherent_accumulations=2)
+    assert (plan.num_blocks, plan.num_doppler_bins, plan.num_secondary_rotations, plan.path) == (80, 2000, 25, 2)
+    assert is_detected(res[0])
+
+
 def test_batch_of_uneven_requests_matches_single_calls():
     """fmdbzp_acquire_batch over three plans of different geometry (L1 C/A at 5 and
     16.368 MHz, L5I rotation search) returns exactly the single-call results."""
