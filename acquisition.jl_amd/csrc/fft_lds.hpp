@@ -243,12 +243,14 @@ __host__ __device__ __forceinline__ int lds_pad_if(int i) { return PAD ? lds_pad
 
 template <bool PAD>
 struct LdsIn {
+    static constexpr bool kLds = true, kPad = PAD;
     const float2* buf;
     int bst;
     __device__ __forceinline__ float2 operator()(int t, int i) const { return buf[t * bst + lds_pad_if<PAD>(i)]; }
 };
 template <bool PAD>
 struct LdsOut {
+    static constexpr bool kLds = true, kPad = PAD;
     float2* buf;
     int bst;
     __device__ __forceinline__ void operator()(int t, int i, float2 v) const
@@ -256,6 +258,31 @@ struct LdsOut {
         buf[t * bst + lds_pad_if<PAD>(i)] = v;
     }
 };
+
+// LdsIn / LdsOut are recognised by the compile-time stages, which then address
+// a butterfly's R elements from one padded base. For a padded buffer,
+// lds_pad(b + 16 m) = lds_pad(b) + 17 m, and lds_pad(jR + r) = lds_pad(jR) + r
+// when R divides 16 and r < R.
+template <class T, class = void>
+struct LdsTraits {
+    static constexpr bool lds = false, pad = false;
+};
+template <class T>
+struct LdsTraits<T, decltype((void)T::kLds)> {
+    static constexpr bool lds = T::kLds, pad = T::kPad;
+};
+// Element step between r-consecutive members at index stride S (S a multiple
+// of 16, or S == 1 with R | 16 and the group starting at a multiple of R).
+template <bool PAD, int S>
+__host__ __device__ constexpr int lds_group_step()
+{
+    return !PAD ? S : (S % 16 == 0 ? S + S / 16 : 1);
+}
+template <bool PAD, int S, int R>
+__host__ __device__ constexpr bool lds_group_ok()
+{
+    return !PAD || S % 16 == 0 || (S == 1 && 16 % R == 0);
+}
 
 // i mod d for the Stockham sub-transform index: a mask when d is a power of
 // two (uniform branch), else a remainder.
@@ -526,16 +553,28 @@ __device__ __forceinline__ void stage_static(const In& in, const Out& out, const
         const int t = idx / M, j = idx % M;
         const int k = j % NS;
         float2 v[R];
+        if constexpr (LdsTraits<In>::lds && lds_group_ok<LdsTraits<In>::pad, M, R>() && M % 16 == 0) {
+            const float2* src = in.buf + t * in.bst + lds_pad_if<LdsTraits<In>::pad>(j);
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = in(t, j + r * M);
+            for (int r = 0; r < R; r++) v[r] = src[r * lds_group_step<LdsTraits<In>::pad, M>()];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = in(t, j + r * M);
+        }
         if (NS > 1) {
 #pragma unroll
             for (int r = 1; r < R; r++) v[r] = cmulf(v[r], twid<INV>(tw, r * k * TSTRIDE));
         }
         bfly<R, INV>(v);
         const int base = (j - k) * R + k;
+        if constexpr (LdsTraits<Out>::lds && lds_group_ok<LdsTraits<Out>::pad, NS, R>()) {
+            float2* dst = out.buf + t * out.bst + lds_pad_if<LdsTraits<Out>::pad>(base);
 #pragma unroll
-        for (int r = 0; r < R; r++) out(t, base + r * NS, v[r]);
+            for (int r = 0; r < R; r++) dst[r * lds_group_step<LdsTraits<Out>::pad, NS>()] = v[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) out(t, base + r * NS, v[r]);
+        }
     }
 }
 
@@ -554,8 +593,14 @@ __device__ __forceinline__ void stage_static_inplace(float2* a, const float2* __
         const int idx = tid + it * NTHR;
         if (TOTAL % NTHR == 0 || idx < TOTAL) {
             const int t = idx / M, j = idx % M;
+            if constexpr (M % 16 == 0) {
+                const float2* src = a + t * BST + lds_pad(j);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[it][r] = a[t * BST + lds_pad(j + r * M)];
+                for (int r = 0; r < R; r++) v[it][r] = src[r * lds_group_step<true, M>()];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) v[it][r] = a[t * BST + lds_pad(j + r * M)];
+            }
         }
     }
     __syncthreads();
@@ -571,8 +616,14 @@ __device__ __forceinline__ void stage_static_inplace(float2* a, const float2* __
             }
             bfly<R, INV>(v[it]);
             const int base = (j - k) * R + k;
+            if constexpr (lds_group_ok<true, NS, R>()) {
+                float2* dst = a + t * BST + lds_pad(base);
 #pragma unroll
-            for (int r = 0; r < R; r++) a[t * BST + lds_pad(base + r * NS)] = v[it][r];
+                for (int r = 0; r < R; r++) dst[r * lds_group_step<true, NS>()] = v[it][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) a[t * BST + lds_pad(base + r * NS)] = v[it][r];
+            }
         }
     }
     __syncthreads();

@@ -519,7 +519,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // every scatter into isolated 4-byte updates; those plans keep the
     // accumulator column-major, with rows contiguous per code column.
     P->acc_cm = !P->simple_kernel && LB < 16;
-    P->rot2000 = P->rot_direct && ndop == 2000;
+    P->rot2000 = P->rot_direct && ndop == 2000 && d.num_coh <= 32;
     if (P->rot2000) {
         P->acc_cm = true;
         LB = P->LB = 1;
@@ -529,7 +529,9 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     P->red_cw = 1024;
     P->red_nchunk = (spc_eff + P->red_cw - 1) / P->red_cw;
     P->col_lds = per_col * LB + fixed;
-    if (P->rot2000) P->col_lds = (size_t)(2000 + 2 * fft_lds_span(2000)) * sizeof(float2);
+    if (P->rot2000)
+        P->col_lds = (size_t)(2000 + 2 * fft_lds_span(2000)) * sizeof(float2) +
+                     ((size_t)d.num_secondary_rotations * 4 + 7) / 8 * 8;
     if (P->col_lds > kLdsMax) return fail(set_err(FMDBZP_EINVAL, "column stage does not fit LDS"));
 
     if ((rc = set_lds_attr((const void*)k_code_fft, P->fft_lds)) ||

@@ -658,66 +658,133 @@ k_column_rotation(ColumnArgs A)
 
 // Direct rotation path at ndop = 2000 (2000 = 16 x 5 x 5 x 5, compile-time):
 // one column per workgroup, two rotation slices per pass.
-struct RotPat2In {
-    const float2* x;
-    const float* pat;  // patterns of this PRN, [Q][ncoh]
-    int q0, nq, ncoh, eff, nb;
-    float inv_nb;
+//
+// Input of rotation q at row n: the column rotated by its alignment offset
+// (held rotated in LDS) times pattern[q][n / nb]. The pattern signs are
+// bitmasks (bit p set: period p negated) and the sign is applied by flipping
+// the float sign bit, which equals the multiplication by -1. The period of
+// row n is umulhi(n, ceil(2^32 / nb)), exact for n, nb < 2^11.
+struct RotSignIn {
+    const float2* xr;
+    uint32_t m0, m1, magic;
     __device__ __forceinline__ float2 operator()(int t, int n) const
     {
-        int src = eff + n;
-        if (src >= 2000) src -= 2000;
-        const int q = t < nq ? q0 + t : q0;
-        int rem;
-        const float sg = pat[q * ncoh + fdivmod(n, nb, inv_nb, &rem)];
-        const float2 u = x[src];
-        return make_float2(sg * u.x, sg * u.y);
+        const uint32_t per = __umulhi((uint32_t)n, magic);
+        const uint32_t s = (((t ? m1 : m0) >> per) & 1u) << 31;
+        const float2 u = xr[n];
+        return make_float2(__uint_as_float(__float_as_uint(u.x) ^ s), __uint_as_float(__float_as_uint(u.y) ^ s));
     }
 };
-struct RotAcc2Out {
-    const ColumnArgs* A;
-    int p, cp, q0, nq;
-    __device__ __forceinline__ void operator()(int t, int w, float2 z) const
-    {
-        if (t >= nq) return;
-        int row = w + 1000;
-        if (row >= 2000) row -= 2000;
-        acc_cell(*A, p, row, cp, q0 + t, fmaf(z.x, z.x, __fmul_rn(z.y, z.y)));
-    }
-};
-
-// grid (bs, nb, batch); dynamic LDS: the column (2000 complex) + two padded
-// transform buffers
+// grid (bs, nb, batch); dynamic LDS: the rotated column (2000 complex), two
+// padded transform buffers and the R pattern masks. Column-major accumulator.
+//
+// The last radix-5 stage is written out here: each thread's 20 output cells
+// (rows of its butterflies, moved by the drift of their destination rows)
+// are fixed for the column, so their offsets are computed once, and the
+// accumulator reads of a pass are all issued before the pass's transforms,
+// instead of one dependent read-add-write per output.
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_column_rot2000(ColumnArgs A)
 {
     constexpr int N = 2000;
+    constexpr int BST = N + (N >> 4) + 1;
+    constexpr int M3 = N / 5;                                    // last stage: radix 5, stride 400
+    constexpr int IT3 = (2 * M3 + FMDBZP_NTHR - 1) / FMDBZP_NTHR;  // butterflies per thread
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int lbk = xcd_logical_block();
     const int lag = lbk % gridDim.x, r = (lbk / gridDim.x) % gridDim.y, p = lbk / (gridDim.x * gridDim.y);
-    float2* x = lds;
+    float2* xr = lds;
     float2* a = lds + N;
-    const float2* T = A.tile + (((size_t)p * A.nb + r) * N) * A.tp + lag;
-    for (int k = tid; k < N; k += FMDBZP_NTHR) x[k] = T[(size_t)k * A.tp];
-    const float* pat = A.patterns + (size_t)A.pset[A.req[p]] * A.ncoh * A.Q;
+    uint32_t* smask = (uint32_t*)(lds + N + 2 * BST);
     const int pc = r * A.bs + lag;
     // per-column alignment to the primary-code wrap (src/noncoherent_integration.jl:212-215)
     const int wrap = ((A.nb - pc / A.bs) % A.nb) * A.bs + (pc % A.bs);
     const int eff = ((wrap + (A.bs >> 1)) / A.bs) % A.nb;
-    const float inv_nb = 1.0f / (float)A.nb;
+    const float2* T = A.tile + (((size_t)p * A.nb + r) * N) * A.tp + lag;
+    for (int k = tid; k < N; k += FMDBZP_NTHR) {
+        int src = k + eff;
+        if (src >= N) src -= N;
+        xr[k] = T[(size_t)src * A.tp];
+    }
+    if (tid < A.R) {
+        const float* pat = A.patterns + ((size_t)A.pset[A.req[p]] * A.Q + tid) * A.ncoh;
+        uint32_t m = 0;
+        for (int q = 0; q < A.ncoh; q++)
+            if (pat[q] < 0.f) m |= 1u << q;
+        smask[tid] = m;
+    }
+    // output cell offsets within a slice: raw bin w = j + 400 i goes to sorted
+    // row (w + 1000) mod 2000, column cp + drift of that row (mod spc)
+    uint32_t off[IT3][5];
+#pragma unroll
+    for (int it = 0; it < IT3; it++) {
+        const int idx = tid + it * FMDBZP_NTHR;
+        const int j = idx % M3;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            int row = j + M3 * i + N / 2;
+            if (row >= N) row -= N;
+            int dc = pc + A.shifts[row];
+            if (dc >= A.spc) dc -= A.spc;
+            off[it][i] = (uint32_t)(row + dc * A.acc_ldc);
+        }
+    }
+    const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)A.nb - 1) / (uint64_t)A.nb);
+    float* accp = A.acc + (size_t)p * A.acc_ps;
+    const size_t slice_stride = (size_t)A.spc * A.acc_ldc;
+    const bool first = A.first_step != 0;
 #pragma unroll 1
     for (int q0 = 0; q0 < A.R; q0 += 2) {
-        __syncthreads();  // x ready; the previous pass is done with the buffers
+        const int nq = min(2, A.R - q0);
+        float* base0 = accp + (size_t)q0 * slice_stride;
+        float prev[IT3][5];
+        if (!first) {
+#pragma unroll
+            for (int it = 0; it < IT3; it++) {
+                const int idx = tid + it * FMDBZP_NTHR;
+                const int t = idx / M3;
+                if (idx < 2 * M3 && t < nq) {
+#pragma unroll
+                    for (int i = 0; i < 5; i++) prev[it][i] = (t ? base0 + slice_stride : base0)[off[it][i]];
+                }
+            }
+        }
+        __syncthreads();  // column and masks ready; the previous pass is done with the buffers
         // An opaque copy of the thread index keeps the transform's index
         // arithmetic and twiddle loads inside the loop: hoisted out of it,
         // they would all stay live in registers across passes.
         int tid_pass;
         asm volatile("v_mov_b32 %0, %1" : "=v"(tid_pass) : "v"(tid));
-        const int nq = min(2, A.R - q0);
-        fft4_static<false, N, 2, FMDBZP_NTHR, 16, 5, 5, 5>(a, A.twD, tid_pass,
-                                                            RotPat2In{x, pat, q0, nq, A.ncoh, eff, A.nb, inv_nb},
-                                                            RotAcc2Out{&A, p, pc, q0, nq});
+        const uint32_t m0 = smask[q0], m1 = smask[nq > 1 ? q0 + 1 : q0];
+        stage_static<16, false, N, 1, 2, FMDBZP_NTHR>(RotSignIn{xr, m0, m1, magic}, LdsOut<true>{a, BST}, A.twD,
+                                                      tid_pass);
+        __syncthreads();
+        stage_static_inplace<5, false, N, 16, 2, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
+        stage_static_inplace<5, false, N, 80, 2, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
+#pragma unroll
+        for (int it = 0; it < IT3; it++) {
+            const int idx = tid_pass + it * FMDBZP_NTHR;
+            if (idx < 2 * M3) {
+                const int t = idx / M3, j = idx - t * M3;
+                float2 v[5];
+                const float2* src = a + t * BST + lds_pad(j);
+#pragma unroll
+                for (int i = 0; i < 5; i++) v[i] = src[i * lds_group_step<true, M3>()];
+#pragma unroll
+                for (int i = 1; i < 5; i++) v[i] = cmulf(v[i], twid<false>(A.twD, i * j));
+                bfly<5, false>(v);
+                if (t < nq) {
+                    float* b = t ? base0 + slice_stride : base0;
+#pragma unroll
+                    for (int i = 0; i < 5; i++) {
+                        // muladd(re, re, im * im), the reference's rotation power
+                        const float pw = fmaf(v[i].x, v[i].x, __fmul_rn(v[i].y, v[i].y));
+                        b[off[it][i]] = first ? pw : prev[it][i] + pw;
+                    }
+                }
+            }
+        }
     }
 }
 
