@@ -629,6 +629,54 @@ __device__ __forceinline__ void stage_static_inplace(float2* a, const float2* __
     __syncthreads();
 }
 
+// In place on two padded LDS transforms BST apart, butterfly position j of
+// both at once: index arithmetic, LDS bases (the second transform is a
+// constant offset) and twiddles are shared by the pair.
+template <int R, bool INV, int N, int NS, int NTHR, int BST>
+__device__ __forceinline__ void stage_static_inplace_pair(float2* a, const float2* __restrict__ tw, int tid)
+{
+    constexpr int M = N / R;
+    constexpr int ITERS = (M + NTHR - 1) / NTHR;
+    constexpr int TSTRIDE = N / (NS * R);
+    static_assert(M % 16 == 0 && lds_group_ok<true, NS, R>(), "grouped padded addressing");
+    float2 v[ITERS][2][R];
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const int j = tid + it * NTHR;
+        if (M % NTHR == 0 || j < M) {
+            const float2* src = a + lds_pad(j);
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int r = 0; r < R; r++) v[it][u][r] = src[u * BST + r * lds_group_step<true, M>()];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+        const int j = tid + it * NTHR;
+        if (M % NTHR == 0 || j < M) {
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    const float2 w = twid<INV>(tw, r * k * TSTRIDE);
+                    v[it][0][r] = cmulf(v[it][0][r], w);
+                    v[it][1][r] = cmulf(v[it][1][r], w);
+                }
+            }
+            bfly<R, INV>(v[it][0]);
+            bfly<R, INV>(v[it][1]);
+            float2* dst = a + lds_pad((j - k) * R + k);
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+#pragma unroll
+                for (int r = 0; r < R; r++) dst[u * BST + r * lds_group_step<true, NS>()] = v[it][u][r];
+        }
+    }
+    __syncthreads();
+}
+
 // Three-stage transform R0 x R1 x R2 = N through gin / gout with one padded
 // LDS buffer per transform. Every thread of the block must call it; the
 // caller places a barrier before it when gin reads LDS written by others.

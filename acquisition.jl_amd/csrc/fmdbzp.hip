@@ -529,6 +529,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     P->red_cw = 1024;
     P->red_nchunk = (spc_eff + P->red_cw - 1) / P->red_cw;
     P->col_lds = per_col * LB + fixed;
+    if (P->col_static) P->col_lds = (size_t)LB * P->bstride * sizeof(float2);  // one column buffer
     if (P->rot2000)
         P->col_lds = (size_t)(2000 + 2 * fft_lds_span(2000)) * sizeof(float2) +
                      ((size_t)d.num_secondary_rotations * 4 + 7) / 8 * 8;

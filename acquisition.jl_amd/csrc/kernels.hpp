@@ -688,8 +688,8 @@ k_column_rot2000(ColumnArgs A)
 {
     constexpr int N = 2000;
     constexpr int BST = N + (N >> 4) + 1;
-    constexpr int M3 = N / 5;                                    // last stage: radix 5, stride 400
-    constexpr int IT3 = (2 * M3 + FMDBZP_NTHR - 1) / FMDBZP_NTHR;  // butterflies per thread
+    constexpr int M3 = N / 5;                                // last stage: radix 5, stride 400
+    constexpr int IT3 = (M3 + FMDBZP_NTHR - 1) / FMDBZP_NTHR;  // butterfly positions per thread
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int lbk = xcd_logical_block();
@@ -719,8 +719,7 @@ k_column_rot2000(ColumnArgs A)
     uint32_t off[IT3][5];
 #pragma unroll
     for (int it = 0; it < IT3; it++) {
-        const int idx = tid + it * FMDBZP_NTHR;
-        const int j = idx % M3;
+        const int j = min(tid + it * FMDBZP_NTHR, M3 - 1);
 #pragma unroll
         for (int i = 0; i < 5; i++) {
             int row = j + M3 * i + N / 2;
@@ -738,15 +737,18 @@ k_column_rot2000(ColumnArgs A)
     for (int q0 = 0; q0 < A.R; q0 += 2) {
         const int nq = min(2, A.R - q0);
         float* base0 = accp + (size_t)q0 * slice_stride;
-        float prev[IT3][5];
+        float* base1 = base0 + slice_stride;
+        float prev[IT3][2][5];
         if (!first) {
 #pragma unroll
             for (int it = 0; it < IT3; it++) {
-                const int idx = tid + it * FMDBZP_NTHR;
-                const int t = idx / M3;
-                if (idx < 2 * M3 && t < nq) {
+                if (tid + it * FMDBZP_NTHR < M3) {
 #pragma unroll
-                    for (int i = 0; i < 5; i++) prev[it][i] = (t ? base0 + slice_stride : base0)[off[it][i]];
+                    for (int i = 0; i < 5; i++) prev[it][0][i] = base0[off[it][i]];
+                    if (nq > 1) {
+#pragma unroll
+                        for (int i = 0; i < 5; i++) prev[it][1][i] = base1[off[it][i]];
+                    }
                 }
             }
         }
@@ -760,27 +762,35 @@ k_column_rot2000(ColumnArgs A)
         stage_static<16, false, N, 1, 2, FMDBZP_NTHR>(RotSignIn{xr, m0, m1, magic}, LdsOut<true>{a, BST}, A.twD,
                                                       tid_pass);
         __syncthreads();
-        stage_static_inplace<5, false, N, 16, 2, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
-        stage_static_inplace<5, false, N, 80, 2, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
+        stage_static_inplace_pair<5, false, N, 16, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
+        stage_static_inplace_pair<5, false, N, 80, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
 #pragma unroll
         for (int it = 0; it < IT3; it++) {
-            const int idx = tid_pass + it * FMDBZP_NTHR;
-            if (idx < 2 * M3) {
-                const int t = idx / M3, j = idx - t * M3;
-                float2 v[5];
-                const float2* src = a + t * BST + lds_pad(j);
+            const int j = tid_pass + it * FMDBZP_NTHR;
+            if (j < M3) {
+                float2 v[2][5];
+                const float2* src = a + lds_pad(j);
 #pragma unroll
-                for (int i = 0; i < 5; i++) v[i] = src[i * lds_group_step<true, M3>()];
+                for (int u = 0; u < 2; u++)
 #pragma unroll
-                for (int i = 1; i < 5; i++) v[i] = cmulf(v[i], twid<false>(A.twD, i * j));
-                bfly<5, false>(v);
-                if (t < nq) {
-                    float* b = t ? base0 + slice_stride : base0;
+                    for (int i = 0; i < 5; i++) v[u][i] = src[u * BST + i * lds_group_step<true, M3>()];
 #pragma unroll
-                    for (int i = 0; i < 5; i++) {
-                        // muladd(re, re, im * im), the reference's rotation power
-                        const float pw = fmaf(v[i].x, v[i].x, __fmul_rn(v[i].y, v[i].y));
-                        b[off[it][i]] = first ? pw : prev[it][i] + pw;
+                for (int i = 1; i < 5; i++) {
+                    const float2 w = twid<false>(A.twD, i * j);
+                    v[0][i] = cmulf(v[0][i], w);
+                    v[1][i] = cmulf(v[1][i], w);
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    if (u < nq) {
+                        bfly<5, false>(v[u]);
+                        float* b = u ? base1 : base0;
+#pragma unroll
+                        for (int i = 0; i < 5; i++) {
+                            // muladd(re, re, im * im), the reference's rotation power
+                            const float pw = fmaf(v[u][i].x, v[u][i].x, __fmul_rn(v[u][i].y, v[u][i].y));
+                            b[off[it][i]] = first ? pw : prev[it][u][i] + pw;
+                        }
                     }
                 }
             }
