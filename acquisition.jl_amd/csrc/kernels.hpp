@@ -320,7 +320,7 @@ k_column_simple(ColumnArgs A)
 // of those cells before the tile loads, so both reads are in flight together
 // and the accumulate needs no second memory round trip.
 template <int LBS, int NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT, 6)  // 6 waves per SIMD: three 512-thread workgroups per CU
 k_column_c160(ColumnArgs A)
 {
     constexpr int N = 160, R0 = 10, M0 = N / R0, R1 = 16, M1 = N / R1;
