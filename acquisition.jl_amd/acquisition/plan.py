@@ -311,7 +311,8 @@ def plan_acquire(system: GNSSSystem, sampling_freq: float, prns: Sequence[int], 
                  use_secondary_code: bool = True,
                  max_secondary_code_rotations: int = 32,
                  fft_flag=None,
-                 literal_edges: bool = False) -> AcquisitionPlan:
+                 literal_edges: bool = False,
+                 tile_precision: str = "fp32") -> AcquisitionPlan:
     """Pre-compute an FM-DBZP acquisition plan (src/plan.jl:312-324).
 
     Keyword arguments and validation errors are the reference's; frequencies
@@ -324,6 +325,11 @@ def plan_acquire(system: GNSSSystem, sampling_freq: float, prns: Sequence[int], 
     one of them. True evaluates all `bit_edge_search_steps` offsets as the
     reference does (src/noncoherent_integration.jl:670-685); results agree to
     rounding.
+
+    `tile_precision` (engine option): "fp32" (default) or "fp16", the mixed
+    form that stores the coherent tile between the block correlation and the
+    column FFT in FP16 under an exact power-of-two scale (integer samples and
+    160-bin simple-path plans only); all arithmetic stays FP32.
     """
     from . import _native
 
@@ -336,6 +342,9 @@ def plan_acquire(system: GNSSSystem, sampling_freq: float, prns: Sequence[int], 
                       use_secondary_code=use_secondary_code,
                       max_secondary_code_rotations=max_secondary_code_rotations)
     plan.literal_edges = bool(literal_edges)
+    if tile_precision not in ("fp32", "fp16"):
+        raise ArgumentError(f"tile_precision must be 'fp32' or 'fp16', got {tile_precision!r}")
+    plan.tile_precision = tile_precision
     plan.native = _native.NativePlan.create(plan)
     return plan
 
@@ -358,7 +367,7 @@ def plan_desc_fields(plan: AcquisitionPlan) -> dict:
         path=plan.path,
         noise_estimator=plan.noise_estimator_id,
         samples_per_chip=plan.samples_per_chip,
-        precision=0,
+        precision=1 if getattr(plan, "tile_precision", "fp32") == "fp16" else 0,
         options=OPT_LITERAL_EDGES if getattr(plan, "literal_edges", False) else 0,
         sampling_freq_hz=plan.sampling_freq,
     )

@@ -17,6 +17,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <cmath>
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -181,6 +182,7 @@ struct fmdbzp_plan {
     int tp = 0, ap = 0;          // tile / accumulator row pitches (multiples of 32 elements)
     bool col_onebuf = false;     // simple column kernel runs its FFT stages in place
     bool col_static = false;     // simple column kernel uses the compile-time ndop = 160 transform
+    bool tile_half = false;      // FMDBZP_PREC_TILE_FP16: FP16 coherent tile
     bool rot_direct = false;     // rotation kernel writes transform outputs straight to the accumulator
     bool rot2000 = false;        // direct rotation at ndop 2000: compile-time kernel, one column per workgroup
     int corr_variant = 0;        // k_corr_tile: 3 = compile-time F = 2048, 2 = two in-place transforms per
@@ -298,7 +300,8 @@ int validate_desc(const fmdbzp_plan_desc* d)
         return set_err(FMDBZP_EINVAL, "rotations > 1 only on the rotation path");
     if (d->noise_estimator != FMDBZP_NOISE_OPPOSITE_ROW && d->noise_estimator != FMDBZP_NOISE_GLOBAL_MEAN)
         return set_err(FMDBZP_EINVAL, "bad noise estimator %d", d->noise_estimator);
-    if (d->precision != FMDBZP_PREC_FP32) return set_err(FMDBZP_EINVAL, "unsupported precision %d", d->precision);
+    if (d->precision != FMDBZP_PREC_FP32 && d->precision != FMDBZP_PREC_TILE_FP16)
+        return set_err(FMDBZP_EINVAL, "unsupported precision %d", d->precision);
     if (d->sampling_freq_hz <= 0) return set_err(FMDBZP_EINVAL, "sampling_freq_hz <= 0");
     // LDS: the F-point transforms need two F-entry buffers in one workgroup
     if (((size_t)2 * fft_lds_span(d->fft_size) + d->fft_size) * sizeof(float2) > kLdsMax)
@@ -514,6 +517,10 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // column kernel with a fixed column count (k_column_c160)
     P->col_static = P->simple_kernel && ndop == 160 && bs >= 32;
     if (P->col_static) LB = 32;
+    P->tile_half = d.precision == FMDBZP_PREC_TILE_FP16;
+    if (P->tile_half && !P->col_static)
+        return fail(set_err(FMDBZP_EINVAL, "FP16 tiles need a simple-path plan with 160 Doppler bins and "
+                                           "block size >= 32"));
     P->LB = LB;
     // With fewer than 16 columns per workgroup, a row-major accumulator turns
     // every scatter into isolated 4-byte updates; those plans keep the
@@ -541,7 +548,12 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         (rc = set_lds_attr((const void*)k_corr_tile<2, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, false>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<2, true, 2048, __half2>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<2, true, 0, __half2>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<1, true, 0, __half2>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<1, false, 0, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_column_c160<32, 512>, P->col_lds)) ||
+        (rc = set_lds_attr((const void*)k_column_c160<32, 512, true>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_simple<true>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_simple<false>, P->col_lds)) ||
         (rc = set_lds_attr((const void*)k_column_sign, P->col_lds)) ||
@@ -663,6 +675,25 @@ int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launc
 
 namespace {
 
+// The block-correlation kernel variant of the plan, writing an FP32 or an FP16 tile.
+template <class TileT>
+void launch_corr(fmdbzp_plan* P, dim3 g, hipStream_t st, const float2* sf, const int32_t* req, TileT* tile, int nb,
+                 int bs, int ndop, float hscale)
+{
+    if (P->corr_variant == 3)
+        k_corr_tile<2, true, 2048, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+    else if (P->corr_variant == 2)
+        k_corr_tile<2, true, 0, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+    else if (P->corr_variant == 1)
+        k_corr_tile<1, true, 0, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+    else
+        k_corr_tile<1, false, 0, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+}
+
 // Enqueue one search on the plan's stream; statistics land in the plan's
 // pinned staging buffer and are returned by acquire_finish. The caller holds
 // the plan's mutex.
@@ -759,6 +790,24 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
     A.rot_direct = P->rot_direct ? 1 : 0;
     const int streamed_rot = (nnc == 1 && d.path == FMDBZP_PATH_ROTATION) ? 1 : 0;
     const int nlb = (bs + P->LB - 1) / P->LB;
+    // FP16 tile: scale 2^-e with bs * max|sample| * 2^-e below the FP16 range
+    // (every tile value is a sum of bs code-weighted samples of that modulus)
+    float hscale = 0.f, hinv = 1.f;
+    if (P->tile_half) {
+        double maxabs = 0.0;
+        switch (eltype) {
+            case FMDBZP_I16: maxabs = 32768.0; break;
+            case FMDBZP_CI8: maxabs = 128.0 * 1.4142135623730951; break;
+            case FMDBZP_CI16: maxabs = 32768.0 * 1.4142135623730951; break;
+            case FMDBZP_C2BIT: maxabs = 3.0 * 1.4142135623730951; break;
+            default:
+                return set_err(FMDBZP_EINVAL, "FP16 tiles need integer samples (I16, CI8, CI16, C2BIT)");
+        }
+        int e = 0;
+        while (bs * maxabs * 1.01 / std::ldexp(1.0, e) > 60000.0) e++;
+        hscale = (float)std::ldexp(1.0, -e);
+        hinv = (float)std::ldexp(1.0, e);
+    }
 
     // fork the lanes off the plan's stream (signal FFT cache, request list)
     HIP_TRY(hipEventRecord(P->ev_fork, P->stream));
@@ -789,18 +838,10 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                     Timed t(P, FMDBZP_K_CORR_TILE, st);
                     const dim3 g(ndop, cnt);
                     const float2* sf = P->d_sigfft + (size_t)s * ndop * F;
-                    if (P->corr_variant == 3)
-                        k_corr_tile<2, true, 2048><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-                            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
-                    else if (P->corr_variant == 2)
-                        k_corr_tile<2, true><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-                            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
-                    else if (P->corr_variant == 1)
-                        k_corr_tile<1, true><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-                            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
+                    if (P->tile_half)
+                        launch_corr(P, g, st, sf, req, (__half2*)tile, nb, bs, ndop, hscale);
                     else
-                        k_corr_tile<1, false><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-                            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f);
+                        launch_corr(P, g, st, sf, req, tile, nb, bs, ndop, 0.f);
                     HIP_TRY(hipGetLastError());
                 }
                 A.shifts = P->d_shifts + (size_t)s * ndop;
@@ -809,8 +850,10 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                     Timed t(P, FMDBZP_K_COLUMN, st);
                     const dim3 g(nlb, nb, cnt);
                     if (P->simple_kernel) {
-                        if (P->col_static)
-                            k_column_c160<32, 512><<<g, 512, P->col_lds, st>>>(A);
+                        if (P->col_static && P->tile_half)
+                            k_column_c160<32, 512, true><<<g, 512, P->col_lds, st>>>(A, hinv);
+                        else if (P->col_static)
+                            k_column_c160<32, 512><<<g, 512, P->col_lds, st>>>(A, 1.f);
                         else if (P->col_onebuf)
                             k_column_simple<true><<<g, FMDBZP_COL_NTHR, P->col_lds, st>>>(A);
                         else
@@ -908,6 +951,9 @@ int validate_request(fmdbzp_plan* P, const void* signal, int32_t eltype, int32_t
     if (!signal || !prn_index || !out_stats) return set_err(FMDBZP_EINVAL, "null argument to fmdbzp_acquire");
     if (nreq <= 0) return set_err(FMDBZP_EINVAL, "nreq must be positive");
     if (!elsize(eltype) && eltype != FMDBZP_C2BIT) return set_err(FMDBZP_EINVAL, "unknown eltype %d", eltype);
+    if (P->tile_half && eltype != FMDBZP_I16 && eltype != FMDBZP_CI8 && eltype != FMDBZP_CI16 &&
+        eltype != FMDBZP_C2BIT)
+        return set_err(FMDBZP_EINVAL, "FP16 tiles need integer samples (I16, CI8, CI16, C2BIT)");
     if ((flags & FMDBZP_FLAG_STORE_SURFACES) && !out_surfaces)
         return set_err(FMDBZP_EINVAL, "STORE_SURFACES without surface pointers");
     for (int i = 0; i < nreq; i++)

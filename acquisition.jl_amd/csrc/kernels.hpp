@@ -9,6 +9,7 @@
 //                                (noncoherent_integration.jl:760-812)
 // p indexes the PRNs of the current batch; the PRN is req[p].
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "fft_lds.hpp"
@@ -183,15 +184,24 @@ struct CorrIn {
         return cmulf(S[i], P[(size_t)b * F + i]);
     }
 };
+// TileT float2: FP32 tile. TileT __half2: FP16 tile of the values times
+// hscale, a power of two (FMDBZP_PREC_TILE_FP16).
+template <class TileT>
 struct LagOut {
-    float2* out;        // tile row of transform 0
+    TileT* out;         // tile row of transform 0
     size_t tstride;     // tile distance between consecutive column blocks
     int bs;
     float inv_f;
     int nt;             // transforms of this pass that exist (the rest are padding)
+    float hscale;
     __device__ __forceinline__ void operator()(int t, int i, float2 v) const
     {
-        if (i < bs && t < nt) out[t * tstride + i] = make_float2(v.x * inv_f, v.y * inv_f);
+        if (i < bs && t < nt) {
+            if constexpr (sizeof(TileT) == sizeof(float2))
+                out[t * tstride + i] = make_float2(v.x * inv_f, v.y * inv_f);
+            else
+                out[t * tstride + i] = __floats2half2_rn((v.x * inv_f) * hscale, (v.y * inv_f) * hscale);
+        }
     }
 };
 
@@ -201,11 +211,11 @@ struct LagOut {
 // (NT = 1).
 // grid (ndop, batch)
 // STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass.
-template <int NT, bool ONEBUF, int STATIC = 0>
+template <int NT, bool ONEBUF, int STATIC = 0, class TileT = float2>
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
-            const int32_t* __restrict__ req, float2* __restrict__ tile, FftPlanDev pf,
-            const float2* __restrict__ twF, int nb, int bs, int tp, int ndop, float inv_f)
+            const int32_t* __restrict__ req, TileT* __restrict__ tile, FftPlanDev pf,
+            const float2* __restrict__ twF, int nb, int bs, int tp, int ndop, float inv_f, float hscale)
 {
     extern __shared__ float2 lds[];
     const int F = pf.n;
@@ -227,7 +237,7 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
         int b0 = kb + r0;
         if (b0 >= nb) b0 -= nb;
         CorrIn in{srow, Pp, F, nb, b0};
-        LagOut out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (size_t)ndop * tp, bs, inv_f, nt};
+        LagOut<TileT> out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (size_t)ndop * tp, bs, inv_f, nt, hscale};
         if constexpr (STATIC == 2048)
             fft3_static<true, 2048, NT, FMDBZP_NTHR, 16, 16, 8>(a, twF, tid, in, out);
         else if constexpr (ONEBUF)
@@ -319,9 +329,9 @@ k_column_simple(ColumnArgs A)
 // radix-16 butterfly of one column (16 accumulator cells); it issues the loads
 // of those cells before the tile loads, so both reads are in flight together
 // and the accumulate needs no second memory round trip.
-template <int LBS, int NT>
+template <int LBS, int NT, bool HALF = false>
 __global__ void __launch_bounds__(NT, 6)  // 6 waves per SIMD: three 512-thread workgroups per CU
-k_column_c160(ColumnArgs A)
+k_column_c160(ColumnArgs A, float hinv)
 {
     constexpr int N = 160, R0 = 10, M0 = N / R0, R1 = 16, M1 = N / R1;
     extern __shared__ float2 lds[];
@@ -358,8 +368,18 @@ k_column_c160(ColumnArgs A)
             const int t = idx % LBS, j = idx / LBS;
             if (t < ncol) {
                 float2 v[R0];
+                if constexpr (HALF) {
+                    // FP16 tile (FMDBZP_PREC_TILE_FP16): back to FP32, times 2^e exactly
+                    const __half2* Th = (const __half2*)A.tile + (((size_t)p * A.nb + r) * N) * A.tp + lag0;
 #pragma unroll
-                for (int m = 0; m < R0; m++) v[m] = T[(size_t)(j + M0 * m) * A.tp + t];
+                    for (int m = 0; m < R0; m++) {
+                        const float2 u = __half22float2(Th[(size_t)(j + M0 * m) * A.tp + t]);
+                        v[m] = make_float2(u.x * hinv, u.y * hinv);
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < R0; m++) v[m] = T[(size_t)(j + M0 * m) * A.tp + t];
+                }
                 bfly10<false>(v);
 #pragma unroll
                 for (int m = 0; m < R0; m++) lds[t * bst + j * R0 + m] = v[m];

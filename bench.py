@@ -229,10 +229,13 @@ def main():
     ap.add_argument("--literal-edges", action="store_true",
                     help="evaluate every bit-edge offset (the default evaluates one: with one data bit per "
                          "coherent block the offsets are circular rotations of the column)")
-    ap.add_argument("--workload", choices=("c2", "c1", "c4", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c4", "c5"), default="c2",
                     help="c2 (default): GPS L1 C/A 16.368 MHz int16 IF, 32 PRNs, N_coh 10, N_nc 8; "
-                         "c1: GPS L1 C/A 5 MHz, 1 ms, 32 PRNs; c4: E1-like 25 MHz, 36 PRNs, N_coh 25, R 25, N_nc 4; "
+                         "c1: GPS L1 C/A 5 MHz, 1 ms, 32 PRNs; c3: the c2 search on quantised complex samples "
+                         "(--c3-kind); c4: E1-like 25 MHz, 36 PRNs, N_coh 25, R 25, N_nc 4; "
                          "c5: batched mix of uneven requests")
+    ap.add_argument("--c3-kind", choices=("ci8", "2bit"), default="ci8")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP16-tile leg of c2 / c3")
     ap.add_argument("--batch-requests", type=int, default=256)
     ap.add_argument("--batch-group", type=int, default=16, help="c5: at most this many requests per submission")
     ap.add_argument("--batch-group-gb", type=float, default=120.0,
@@ -279,6 +282,15 @@ def main():
                        literal_edges=args.literal_edges)
         workload = ("GPS L1 C/A 16.368 MHz real int16 at IF 4.092 MHz, 80 ms; PRNs 1-32 (8 present); "
                     f"N_coh 10, bit-edge steps {args.edge_steps}, N_nc 8, +-7 kHz")
+    elif args.workload == "c3":
+        from acquisition.signals import l1ca_quantized_scene
+        sc = l1ca_quantized_scene(kind=args.c3_kind)
+        plan_kw = dict(num_coherently_integrated_code_periods=10, bit_edge_search_steps=args.edge_steps,
+                       num_noncoherent_accumulations=8, min_doppler_coverage=7000.0,
+                       literal_edges=args.literal_edges)
+        workload = ("GPS L1 C/A 16.368 MHz complex at IF 4.092 MHz, quantised to "
+                    + ("Complex{Int8} (sigma 32 LSB)" if args.c3_kind == "ci8" else "2-bit {+-1, +-3} packed 4 per byte")
+                    + f", 80 ms; PRNs 1-32 (8 present); N_coh 10, bit-edge steps {args.edge_steps}, N_nc 8, +-7 kHz")
     elif args.workload == "c1":
         from acquisition.signals import l1ca_5mhz_scene
         sc = l1ca_5mhz_scene()
@@ -291,10 +303,15 @@ def main():
                        min_doppler_coverage=7000.0)
         workload = ("E1-like pilots (4092-chip codes, BOC(1,1), 25-chip secondary) at 25 MHz, 4 x 100 ms; "
                     "PRNs 1-36 (6 present); N_coh 25, R 25, N_nc 4, +-7 kHz")
-    log(f"scene: {sc.signal.shape[0]} samples, present {sc.present} ({time.time() - t0:.1f}s)")
+    nsamp = sc.signal.n if hasattr(sc.signal, "n") else sc.signal.shape[0]
+    log(f"scene: {nsamp} samples, present {sc.present} ({time.time() - t0:.1f}s)")
     prns = list(sc.prns)
     plan = plan_acquire(sc.system, sc.sampling_freq, prns, **plan_kw)
-    dev = DeviceArray(sc.signal)
+    from acquisition import Packed2BitSignal
+    if isinstance(sc.signal, Packed2BitSignal):
+        dev = Packed2BitSignal(DeviceArray(sc.signal.packed), sc.signal.n)
+    else:
+        dev = DeviceArray(sc.signal)
     ndop, spc, R = plan.num_doppler_bins, plan.samples_per_code, plan.num_secondary_rotations
     cells = len(prns) * ndop * spc * R
     log(f"plan: spc {spc} nb {plan.num_blocks} bs {plan.block_size} F {plan.fft_size} ndop {ndop} "
@@ -350,6 +367,29 @@ def main():
                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bpl)}
     kernel_share = {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]}
 
+    # mixed-precision leg: the same search with the coherent tile in FP16
+    mixed = None
+    if args.workload in ("c2", "c3") and not args.no_mixed:
+        plan16 = plan_acquire(sc.system, sc.sampling_freq, prns, tile_precision="fp16", **plan_kw)
+        for i in range(args.warmup):
+            res16 = acquire_(plan16, dev, prns, interm_freq=sc.interm_freq)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            res16 = acquire_(plan16, dev, prns, interm_freq=sc.interm_freq)
+        dt16 = (time.perf_counter() - t0) / args.steps
+        plan16.close()
+        same_det = all(is_detected(a) == is_detected(b) for a, b in zip(res, res16))
+        bins = [(a.engine_stats["row"], a.engine_stats["col"]) == (b.engine_stats["row"], b.engine_stats["col"])
+                for a, b in zip(res, res16)]
+        det_bins = all(e for e, a in zip(bins, res) if is_detected(a))
+        mixed = {"form": "FP16 coherent tile under an exact power-of-two scale, FP32 arithmetic",
+                 "value": round(cells / dt16, 1), "ms_per_step": round(dt16 * 1e3, 4),
+                 "speedup_vs_fp32": round(dt / dt16, 3), "detections_equal": same_det,
+                 "peak_bins_equal_detected": det_bins, "peak_bins_equal_all": f"{sum(bins)}/{len(bins)}",
+                 "fp64_oracle_error": "tests/test_gpu_mixed.py: at most 1.8e-4 of the peak, 1.5e-3 per cell"}
+        log(f"mixed (fp16 tile): {dt16 * 1e3:.3f} ms/step, x{dt / dt16:.3f}; detections equal {same_det}, "
+            f"detected peak bins equal {det_bins}, all {sum(bins)}/{len(bins)}")
+
     cpu = None
     if not args.no_cpu_baseline and args.workload != "c4":
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -383,18 +423,20 @@ def main():
                    "num_blocks": nb, "fft_size": F, "num_coh": plan.num_coherently_integrated_code_periods,
                    "bit_edge_search_steps": plan.bit_edge_search_steps, "num_noncoherent": nnc,
                    "secondary_rotations": R, "precision": "fp32",
-                   "signal": f"device-resident {sc.signal.dtype}",
+                   "signal": "device-resident " + ("packed 2-bit" if isinstance(sc.signal, Packed2BitSignal)
+                                                    else str(sc.signal.dtype)),
                    "edge_evaluation": "literal" if args.literal_edges else "one-edge identity"},
         "prn_acqs_per_s": round(len(prns) / dt, 2),
         "host_signal_cells_per_s": round(cells / dt_host, 1),
         "cell_dwells_per_s": round(cells * nnc / dt, 1),
         "kernel_ms_per_step": kernel_share,
         "detected": det,
+        "mixed_precision": mixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    dev.free()
+    (dev.packed if isinstance(dev, Packed2BitSignal) else dev).free()
     plan.close()
 
 

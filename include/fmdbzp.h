@@ -73,7 +73,15 @@ enum fmdbzp_noise_estimator {
 };
 
 enum fmdbzp_precision {
-    FMDBZP_PREC_FP32 = 0      /* plain FP32 path */
+    FMDBZP_PREC_FP32 = 0,     /* plain FP32 path */
+    /* Mixed: the coherent tile (block-correlation output, column-FFT input)
+     * is stored in FP16, scaled by an exact power of two chosen from the
+     * integer sample type's bound so that no tile value can overflow; all
+     * arithmetic stays FP32. Half the tile traffic between the two stages.
+     * Ndop-160 simple-path plans (the L1 C/A 10 ms searches) and integer
+     * samples (I16, CI8, CI16, C2BIT) only; other plans are rejected at
+     * creation and float samples at the search call (FMDBZP_EINVAL). */
+    FMDBZP_PREC_TILE_FP16 = 1
 };
 
 /* Plan options (fmdbzp_plan_desc.options). With N_db == 1 the bit-edge

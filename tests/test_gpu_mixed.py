@@ -1,0 +1,115 @@
+"""Mixed-precision form (FP16 coherent tile) against the FP32 path.
+
+The FP16 tile holds the block-correlation output between the two stages
+under an exact power-of-two scale; every sum, transform and power stays in
+FP32. Each tile value is rounded once to 11 significant bits (relative
+2^-11), so cell powers move by about 1e-3 relative at the noise floor and much
+less at a coherent peak. The bar: identical detection decisions, identical
+peak (row, column) for every detected PRN, surfaces within 2e-3 of the peak
+globally and 2e-2 per cell (cells above 1e-3 of the peak), and the error
+against the FP64 oracle stated.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from acquisition import EngineError, acquire_, is_detected, plan_acquire
+from acquisition.signals import l1ca_if_int16_scene, l1ca_quantized_scene
+
+pytestmark = pytest.mark.gpu
+
+C2_KW = dict(num_coherently_integrated_code_periods=10, bit_edge_search_steps=10, num_noncoherent_accumulations=8,
+             min_doppler_coverage=7000.0)
+
+
+def _both(sc, prns, surfaces=False, **kw):
+    out = {}
+    for prec in ("fp32", "fp16"):
+        plan = plan_acquire(sc.system, sc.sampling_freq, prns, tile_precision=prec, **kw)
+        try:
+            res = acquire_(plan, sc.signal, prns, interm_freq=sc.interm_freq, store_power_bins=surfaces)
+            out[prec] = [(r, np.array(r.power_bins, copy=True) if surfaces else None) for r in res]
+        finally:
+            plan.close()
+    return out["fp32"], out["fp16"]
+
+
+def _check_decisions(r32, r16, present):
+    ndet = 0
+    for (a, _), (b, _) in zip(r32, r16):
+        assert a.prn == b.prn
+        assert is_detected(a) == is_detected(b), f"prn {a.prn}: detection differs"
+        if is_detected(a):
+            ndet += 1
+            sa, sb = a.engine_stats, b.engine_stats
+            assert (sa["row"], sa["col"]) == (sb["row"], sb["col"]), f"prn {a.prn}: peak bin"
+            assert a.code_phase == b.code_phase and a.carrier_doppler == b.carrier_doppler
+            assert abs(b.peak_to_noise_ratio / a.peak_to_noise_ratio - 1) < 2e-3
+    assert ndet >= len(present) // 2
+    return ndet
+
+
+def _surface_err(ref, got):
+    ref = ref.astype(np.float64)
+    got = got.astype(np.float64)
+    peak = ref.max()
+    glob = np.abs(got - ref).max() / peak
+    m = ref >= 1e-3 * peak
+    cell = (np.abs(got - ref)[m] / ref[m]).max()
+    return glob, cell
+
+
+def test_fp16_tile_c2_scene_matches_fp32():
+    """GPS L1 C/A 16.368 MHz int16 IF, 80 ms, PRNs 1-32, N_coh 10, E 10, N_nc 8."""
+    sc = l1ca_if_int16_scene(seed=0x5EED0002)
+    prns = list(range(1, 33))
+    r32, r16 = _both(sc, prns, **C2_KW)
+    ndet = _check_decisions(r32, r16, sc.present)
+    print(f"C2: {ndet} detected, decisions and peak bins identical")
+
+
+def test_fp16_tile_surfaces_and_fp64_error():
+    """Surfaces of two present PRNs and one absent PRN against the FP32 path
+    and against the FP64 oracle."""
+    sc = l1ca_if_int16_scene(seed=0x5EED0002)
+    absent = [p for p in range(1, 33) if p not in sc.present][0]
+    prns = sc.present[:2] + [absent]
+    r32, r16 = _both(sc, prns, surfaces=True, **C2_KW)
+    plan = plan_acquire(sc.system, sc.sampling_freq, prns, **C2_KW)
+    try:
+        _, s64 = oracle.acquire(plan, sc.signal, list(range(len(prns))), if_hz=sc.interm_freq, surfaces=True,
+                                bits=64)
+    finally:
+        plan.close()
+    for j, p in enumerate(prns):
+        g16, c16 = _surface_err(r32[j][1], r16[j][1])
+        g32_64, c32_64 = _surface_err(s64[j], r32[j][1])
+        g16_64, c16_64 = _surface_err(s64[j], r16[j][1])
+        print(f"prn {p}: fp16 vs fp32 {g16:.2e}/{c16:.2e}; vs FP64 oracle: fp32 {g32_64:.2e}/{c32_64:.2e}, "
+              f"fp16 {g16_64:.2e}/{c16_64:.2e} (global / per cell)")
+        assert g16 <= 2e-3 and c16 <= 2e-2
+        assert g16_64 <= 2e-3 and c16_64 <= 2e-2
+
+
+@pytest.mark.parametrize("kind", ["ci8", "2bit"])
+def test_fp16_tile_quantized_inputs(kind):
+    """The quantised C2 scene (Complex{Int8}, 2-bit {+-1, +-3}), 80 ms, PRNs 1-32."""
+    sc = l1ca_quantized_scene(kind=kind)
+    prns = list(range(1, 33))
+    r32, r16 = _both(sc, prns, **C2_KW)
+    ndet = _check_decisions(r32, r16, sc.present)
+    print(f"{kind}: {ndet} detected, decisions and peak bins identical")
+
+
+def test_fp16_tile_rejections():
+    sc = l1ca_if_int16_scene(seed=0x5EED0002, n_ms=20, npresent=2)
+    plan = plan_acquire(sc.system, sc.sampling_freq, [1, 2], tile_precision="fp16",
+                        num_coherently_integrated_code_periods=10, num_noncoherent_accumulations=2)
+    try:
+        with pytest.raises(EngineError, match="integer samples"):
+            acquire_(plan, sc.signal.astype(np.float32), [1, 2], interm_freq=sc.interm_freq)
+        acquire_(plan, sc.signal, [1, 2], interm_freq=sc.interm_freq)  # the plan still works
+    finally:
+        plan.close()
+    with pytest.raises(EngineError, match="160 Doppler bins"):
+        plan_acquire(sc.system, 5e6, [3], tile_precision="fp16")
