@@ -185,7 +185,8 @@ struct fmdbzp_plan {
     bool tile_half = false;      // FMDBZP_PREC_TILE_FP16: FP16 coherent tile
     bool rot_direct = false;     // rotation kernel writes transform outputs straight to the accumulator
     bool rot2000 = false;        // direct rotation at ndop 2000: compile-time kernel, one column per workgroup
-    int corr_variant = 0;        // k_corr_tile: 3 = compile-time F = 2048, 2 = two in-place transforms per
+    int corr_variant = 0;        // k_corr_tile: 4 = compile-time F = 2500, 3 = compile-time F = 2048,
+                                 // 2 = two in-place transforms per
                                  // pass, 1 = one, 0 = ping-pong
     size_t col_lds = 0;
     size_t fft_lds = 0;
@@ -470,11 +471,12 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // stages, one buffer per concurrent transform (two when they fit the
     // register bound), else two ping-pong buffers
     if (F == 2048) P->corr_variant = 3;  // compile-time 16 x 16 x 8 transforms, two per pass
+    else if (F == 2500) P->corr_variant = 4;  // compile-time 10 x 10 x 5 x 5, one per pass
     else if (fft_inplace_ok(P->pf) && 2 * F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 2;
     else if (fft_inplace_ok(P->pf) && F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 1;
     else P->corr_variant = 0;
     P->corr_lds = (size_t)F * sizeof(float2) +
-                  (size_t)(P->corr_variant == 1 ? 1 : 2) * fft_lds_span(F) * sizeof(float2);
+                  (size_t)(P->corr_variant == 1 || P->corr_variant == 4 ? 1 : 2) * fft_lds_span(F) * sizeof(float2);
 
     // column-stage geometry: LB columns per workgroup, padded column stride
     // (odd, so lanes on consecutive columns spread over the LDS banks)
@@ -549,6 +551,8 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         (rc = set_lds_attr((const void*)k_corr_tile<1, true>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, false>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<2, true, 2048, __half2>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<1, true, 2500>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<1, true, 2500, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<2, true, 0, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true, 0, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, false, 0, __half2>, P->corr_lds)) ||
@@ -680,7 +684,10 @@ template <class TileT>
 void launch_corr(fmdbzp_plan* P, dim3 g, hipStream_t st, const float2* sf, const int32_t* req, TileT* tile, int nb,
                  int bs, int ndop, float hscale)
 {
-    if (P->corr_variant == 3)
+    if (P->corr_variant == 4)
+        k_corr_tile<1, true, 2500, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+    else if (P->corr_variant == 3)
         k_corr_tile<2, true, 2048, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
             sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
     else if (P->corr_variant == 2)

@@ -210,7 +210,8 @@ struct LagOut {
 // stages, one LDS buffer per concurrent transform; else two ping-pong buffers
 // (NT = 1).
 // grid (ndop, batch)
-// STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass.
+// STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass; STATIC =
+// 2500: 10 x 10 x 5 x 5 (the E1-like search at 25 MHz).
 template <int NT, bool ONEBUF, int STATIC = 0, class TileT = float2>
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
@@ -240,6 +241,8 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
         LagOut<TileT> out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (size_t)ndop * tp, bs, inv_f, nt, hscale};
         if constexpr (STATIC == 2048)
             fft3_static<true, 2048, NT, FMDBZP_NTHR, 16, 16, 8>(a, twF, tid, in, out);
+        else if constexpr (STATIC == 2500)
+            fft4_static<true, 2500, NT, FMDBZP_NTHR, 10, 10, 5, 5>(a, twF, tid, in, out);
         else if constexpr (ONEBUF)
             fft_run_1buf<true, false, FMDBZP_CORR_MAXV>(a, pf, twF, nt, span, tid, FMDBZP_NTHR, in, out);
         else

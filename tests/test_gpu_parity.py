@@ -299,6 +299,20 @@ def test_e1like_rotation_boc_reduced():
     assert res[0].secondary_code_phase == sc.truth[p0]["secondary_phase"]
 
 
+def test_correlation_f2500():
+    """GPS L1 C/A 5 MHz, +-1 kHz, N_coh 2, N_nc 2: nb 4, bs 1250, F 2500, the
+    block-correlation size of the 25 MHz E1-like search (compile-time
+    10 x 10 x 5 x 5 transform)."""
+    sysm = GPSL1CA()
+    g = generate_test_signal(sysm, 5, num_samples=4 * 5000, sampling_freq=5e6, interm_freq=0.0, CN0=45,
+                             doppler=321.0, code_phase=401.25, seed=23)
+    plan, res, ores = run_case(sysm, 5e6, [5, 6], g["signal"].astype(np.complex64), interp=True, label="f2500",
+                               min_doppler_coverage=1000.0, num_coherently_integrated_code_periods=2,
+                               num_noncoherent_accumulations=2)
+    assert (plan.num_blocks, plan.block_size, plan.fft_size) == (4, 1250, 2500)
+    assert is_detected(res[0])
+
+
 def test_rotation_ndop2000_two_steps():
     """E1-like pilots at 1 MHz, N_coh 25, +-9 kHz: nb 80, ndop 2000, the
     geometry of the 25 MHz E1-like search (whose compile-time 2000-point
