@@ -535,7 +535,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     }
     P->apc = (ndop + 31) & ~31;
     P->acc_ps = P->acc_cm ? (size_t)spc_eff * P->apc : (size_t)ndop * P->ap;
-    P->red_cw = 1024;
+    P->red_cw = 4096;
     P->red_nchunk = (spc_eff + P->red_cw - 1) / P->red_cw;
     P->col_lds = per_col * LB + fixed;
     if (P->col_static) P->col_lds = (size_t)LB * P->bstride * sizeof(float2);  // one column buffer
@@ -880,7 +880,8 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                 if (P->acc_cm) {
                     const size_t po = (size_t)lo * P->red_nchunk * ndop;
                     const int rb = (ndop + FMDBZP_NTHR - 1) / FMDBZP_NTHR;
-                    k_reduce_cols<<<dim3(P->red_nchunk, rb, cnt), FMDBZP_NTHR, 0, st>>>(
+                    const int rb4 = (ndop + 4 * FMDBZP_NTHR - 1) / (4 * FMDBZP_NTHR);
+                    k_reduce_cols<<<dim3(P->red_nchunk, rb4, cnt), FMDBZP_NTHR, 0, st>>>(
                         acc, P->acc_ps, P->apc, ndop, spc, spc_eff, d.num_secondary_rotations, streamed_rot,
                         P->red_cw, P->d_psum + po, P->d_pbest_v + po, P->d_pbest_k + po);
                     HIP_TRY(hipGetLastError());

@@ -883,32 +883,46 @@ k_reduce_rows(const float* __restrict__ acc, int ndop, int spc, int spc_eff, int
     }
 }
 
-// Column-major accumulator, pass 1. grid (column chunks, ceil(ndop / NTHR),
-// batch): each thread owns one row and walks the chunk's `cw` columns in
-// order (consecutive lanes read consecutive rows of a column), producing the
-// chunk's FP64 row sum and best (value, visit-order key).
+// Column-major accumulator, pass 1. grid (column chunks, ceil(ndop / (4 NTHR)),
+// batch): each thread owns 4 consecutive rows (one 16-byte load per column;
+// the column pitch is a multiple of 32 rows, so the load stays in the padded
+// column) and walks the chunk's `cw` columns in order, producing the chunk's
+// FP64 row sums and best (value, visit-order key) per row. A cell's key is
+// formed only when its value reaches the row's best so far.
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_reduce_cols(const float* __restrict__ acc, size_t acc_ps, int ldc, int ndop, int spc, int spc_eff, int R,
               int streamed_rot, int cw, double* __restrict__ psum, float* __restrict__ pbest_v,
               uint64_t* __restrict__ pbest_k)
 {
-    const int chunk = blockIdx.x, row = blockIdx.y * FMDBZP_NTHR + threadIdx.x, p = blockIdx.z;
-    if (row >= ndop) return;
-    const float* a = acc + (size_t)p * acc_ps + row;
+    const int chunk = blockIdx.x, row0 = 4 * (blockIdx.y * FMDBZP_NTHR + threadIdx.x), p = blockIdx.z;
+    if (row0 >= ndop) return;
+    const float* a = acc + (size_t)p * acc_ps + row0;
     const int c0 = chunk * cw, c1 = min(spc_eff, c0 + cw);
-    double s = 0.0;
-    float bv = -INFINITY;
-    uint64_t bk = ~0ull;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    float bv[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    uint64_t bk[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+#pragma unroll 4
     for (int c = c0; c < c1; c++) {
-        const float v = a[(size_t)c * ldc];
-        s += (double)v;
-        const uint64_t k = cell_key(row, c, ndop, spc, R, streamed_rot);
-        if (better(v, k, bv, bk)) { bv = v; bk = k; }
+        const float4 v4 = *(const float4*)(a + (size_t)c * ldc);
+        const float v[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            s[u] += (double)v[u];
+            if (v[u] >= bv[u]) {
+                const uint64_t k = cell_key(row0 + u, c, ndop, spc, R, streamed_rot);
+                if (better(v[u], k, bv[u], bk[u])) { bv[u] = v[u]; bk[u] = k; }
+            }
+        }
     }
-    const size_t o = ((size_t)p * gridDim.x + chunk) * ndop + row;
-    psum[o] = s;
-    pbest_v[o] = bv;
-    pbest_k[o] = bk;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        if (row0 + u < ndop) {
+            const size_t o = ((size_t)p * gridDim.x + chunk) * ndop + row0 + u;
+            psum[o] = s[u];
+            pbest_v[o] = bv[u];
+            pbest_k[o] = bk[u];
+        }
+    }
 }
 
 // Column-major accumulator, pass 2. grid (ceil(ndop / NTHR), batch): per row,
