@@ -124,6 +124,7 @@ EXPORTED_SYMBOLS = (
     "fmdbzp_plan_set_timing",
     "fmdbzp_plan_kernel_times",
     "fmdbzp_synthesize",
+    "fmdbzp_plan_set_surface_layout",
 )
 
 
@@ -151,6 +152,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fmdbzp_plan_set_timing.restype = ctypes.c_int
     lib.fmdbzp_plan_kernel_times.argtypes = [P, P, P, ctypes.c_int32]
     lib.fmdbzp_plan_kernel_times.restype = ctypes.c_int
+    lib.fmdbzp_plan_set_surface_layout.argtypes = [P, ctypes.c_int32, P]
+    lib.fmdbzp_plan_set_surface_layout.restype = ctypes.c_int
     lib.fmdbzp_synthesize.argtypes = [ctypes.POINTER(SceneDesc), P, ctypes.c_int32, P]
     lib.fmdbzp_synthesize.restype = ctypes.c_int
     return lib
@@ -247,6 +250,12 @@ class NativePlan:
                     ctypes.cast(stats, ctypes.c_void_p).value,
                     ctypes.cast(surf_arg, ctypes.c_void_p).value if surf_arg is not None else None)
         return q, (idx, stats, surf_arg)
+
+    def set_surface_layout(self, nbins: int, chip_order=None):
+        order = None if chip_order is None else np.ascontiguousarray(chip_order, dtype=np.int32)
+        _check(lib().fmdbzp_plan_set_surface_layout(self.handle, int(nbins),
+                                                    None if order is None else order.ctypes.data),
+               "fmdbzp_plan_set_surface_layout")
 
     def set_timing(self, enable: bool):
         _check(lib().fmdbzp_plan_set_timing(self.handle, 1 if enable else 0), "fmdbzp_plan_set_timing")

@@ -317,9 +317,10 @@ def _extract(plan: AcquisitionPlan, st, prn: int, sigview: _SignalView, if_hz: f
 def _get_result_buffer(plan: AcquisitionPlan, prn_idx: int) -> np.ndarray:
     """Plan-owned (ndop, spc_eff) Float32 surface, allocated on first use and
     reused (src/acquire.jl:221-228)."""
+    ncols = getattr(plan, "power_bins_chip_bins", 0) or plan.samples_per_code_eff
     buf = plan.result_buffers[prn_idx]
-    if buf is None:
-        buf = np.empty((plan.num_doppler_bins, plan.samples_per_code_eff), dtype=np.float32, order="F")
+    if buf is None or buf.shape[1] != ncols:
+        buf = np.empty((plan.num_doppler_bins, ncols), dtype=np.float32, order="F")
         plan.result_buffers[prn_idx] = buf
     return buf
 

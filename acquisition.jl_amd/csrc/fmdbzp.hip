@@ -175,7 +175,9 @@ struct fmdbzp_plan {
     // the request whose statistics are staged in h_stats, not yet returned
     fmdbzp_prn_stats* pending_out = nullptr;
     int32_t pending_n = 0;
-    float* d_export = nullptr;  // one (ndop, spc_eff) staging surface per lane
+    float* d_export = nullptr;  // one staging surface per lane (ndop x surface columns)
+    int surf_bins = 0;          // 0: scrambled (ndop, spc_eff) surfaces; > 0: chip-ordered (ndop, surf_bins)
+    int32_t* d_chip_order = nullptr;  // [spc] code columns in ascending code phase
     int batch = 1;
     int LB = 1, bstride = 1, sst = 1;
     bool simple_kernel = false;  // column stage runs k_column_simple
@@ -201,6 +203,8 @@ struct fmdbzp_plan {
 };
 
 namespace {
+
+int acquire_finish(fmdbzp_plan* P);
 
 size_t elsize(int eltype)
 {
@@ -360,6 +364,7 @@ void free_plan(fmdbzp_plan* P)
     hipFree(P->d_req);
     hipFree(P->d_stats);
     hipFree(P->d_export);
+    hipFree(P->d_chip_order);
     hipHostFree(P->h_req);
     hipHostFree(P->h_stats);
     hipHostFree(P->h_signal);
@@ -664,6 +669,40 @@ int fmdbzp_plan_set_timing(fmdbzp_plan* plan, int32_t enable)
     return FMDBZP_OK;
 }
 
+int fmdbzp_plan_set_surface_layout(fmdbzp_plan* plan, int32_t nbins, const int32_t* chip_order)
+{
+    g_err.clear();
+    if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
+    std::lock_guard<std::mutex> lk(plan->mu);
+    const int spc = plan->d.samples_per_code;
+    if (nbins < 0 || nbins > spc) return set_err(FMDBZP_EINVAL, "nbins %d outside [0, %d]", nbins, spc);
+    if (nbins > 0) {
+        if (!chip_order) return set_err(FMDBZP_EINVAL, "chip_order is required for nbins > 0");
+        std::vector<char> seen((size_t)spc, 0);
+        for (int i = 0; i < spc; i++) {
+            const int c = chip_order[i];
+            if (c < 0 || c >= spc || seen[(size_t)c]) return set_err(FMDBZP_EINVAL, "chip_order is not a permutation");
+            seen[(size_t)c] = 1;
+        }
+    }
+    HIP_TRY(hipSetDevice(plan->device));
+    if (plan->pending_out) {
+        const int rc = acquire_finish(plan);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(plan->stream));
+    if (nbins > 0) {
+        if (!plan->d_chip_order) HIP_TRY(hipMalloc((void**)&plan->d_chip_order, sizeof(int32_t) * (size_t)spc));
+        HIP_TRY(hipMemcpy(plan->d_chip_order, chip_order, sizeof(int32_t) * (size_t)spc, hipMemcpyHostToDevice));
+    }
+    if (nbins != plan->surf_bins) {  // the staging surfaces change size
+        hipFree(plan->d_export);
+        plan->d_export = nullptr;
+    }
+    plan->surf_bins = nbins;
+    return FMDBZP_OK;
+}
+
 int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launches, int32_t n)
 {
     if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
@@ -759,8 +798,9 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
     }
     memcpy(P->h_req, prn_index, sizeof(int32_t) * nreq);
     HIP_TRY(hipMemcpyAsync(P->d_req, P->h_req, sizeof(int32_t) * nreq, hipMemcpyHostToDevice, P->stream));
+    const size_t surf_cols = P->surf_bins > 0 ? (size_t)P->surf_bins : (size_t)spc_eff;
     if (store && !P->d_export && !(flags & FMDBZP_FLAG_DEVICE_OUTPUT))
-        HIP_TRY(hipMalloc((void**)&P->d_export, sizeof(float) * (size_t)ndop * spc_eff * kLanes));
+        HIP_TRY(hipMalloc((void**)&P->d_export, sizeof(float) * (size_t)ndop * surf_cols * kLanes));
 
     // 1. downconvert + double-block FFTs of every segment
     {
@@ -905,13 +945,19 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                 HIP_TRY(hipGetLastError());
             }
             if (store) {
-                float* stage = dev_out ? nullptr : P->d_export + (size_t)l * ndop * spc_eff;
+                float* stage = dev_out ? nullptr : P->d_export + (size_t)l * ndop * surf_cols;
                 for (int p = 0; p < cnt; p++) {
                     float* dst = out_surfaces[b0 + lo + p];
                     if (!dst) continue;
                     {
                         Timed t(P, FMDBZP_K_EXPORT, st);
-                        if (P->acc_cm) {
+                        if (P->surf_bins > 0) {
+                            dim3 g(P->surf_bins, (ndop + FMDBZP_NTHR - 1) / FMDBZP_NTHR);
+                            k_export_chip<<<g, FMDBZP_NTHR, 0, st>>>(acc + (size_t)p * acc_per_prn, dev_out ? dst : stage,
+                                                                     P->d_chip_order, ndop, spc,
+                                                                     d.num_secondary_rotations, P->surf_bins,
+                                                                     A.acc_ldr, A.acc_ldc);
+                        } else if (P->acc_cm) {
                             dim3 g(spc_eff, (ndop + FMDBZP_NTHR - 1) / FMDBZP_NTHR);
                             k_export_cm<<<g, FMDBZP_NTHR, 0, st>>>(acc + (size_t)p * acc_per_prn,
                                                                    dev_out ? dst : stage, ndop, P->apc);
@@ -923,7 +969,7 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
                         HIP_TRY(hipGetLastError());
                     }
                     if (!dev_out)
-                        HIP_TRY(hipMemcpyAsync(dst, stage, sizeof(float) * (size_t)ndop * spc_eff,
+                        HIP_TRY(hipMemcpyAsync(dst, stage, sizeof(float) * (size_t)ndop * surf_cols,
                                                hipMemcpyDeviceToHost, st));
                 }
             }

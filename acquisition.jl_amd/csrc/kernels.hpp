@@ -1070,6 +1070,27 @@ k_export_cm(const float* __restrict__ acc, float* __restrict__ out, int ndop, in
     if (row < ndop) out[(size_t)col * ndop + row] = acc[(size_t)col * ldc + row];
 }
 
+// Chip-ordered surface export (fmdbzp_plan_set_surface_layout): grid (nbins,
+// ceil(ndop / NTHR)). Bin b holds, per sorted Doppler row, the maximum over
+// the code columns order[b spc / nbins .. (b + 1) spc / nbins) (ascending code
+// phase, src/plot.jl:35-57) and over the R rotation slices of each;
+// out is column-major (ndop, nbins).
+__global__ void __launch_bounds__(FMDBZP_NTHR)
+k_export_chip(const float* __restrict__ acc, float* __restrict__ out, const int32_t* __restrict__ order, int ndop,
+              int spc, int R, int nbins, int ldr, int ldc)
+{
+    const int b = blockIdx.x, row = blockIdx.y * FMDBZP_NTHR + threadIdx.x;
+    if (row >= ndop) return;
+    const int i0 = (int)((int64_t)b * spc / nbins), i1 = (int)((int64_t)(b + 1) * spc / nbins);
+    const float* a = acc + (size_t)row * ldr;
+    float m = -INFINITY;
+    for (int i = i0; i < i1; i++) {
+        const int c = order[i];
+        for (int sl = 0; sl < R; sl++) m = fmaxf(m, a[((size_t)sl * spc + c) * ldc]);
+    }
+    out[(size_t)b * ndop + row] = m;
+}
+
 // Surface export for store_power_bins: acc[row][col] (row pitch ap) -> Julia column-major
 // (ndop, spc_eff), i.e. out[col * ndop + row]. 32x32 LDS-tiled transpose.
 // grid (ceil(spc_eff/32), ceil(ndop/32)), block (32, 8)

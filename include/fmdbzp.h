@@ -250,6 +250,21 @@ enum fmdbzp_kernel_class {
 int fmdbzp_plan_set_timing(fmdbzp_plan* plan, int32_t enable);
 int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launches, int32_t n);
 
+/*
+ * Layout of the surfaces that FMDBZP_FLAG_STORE_SURFACES returns (the plot
+ * recipe's input, src/plot.jl:35-57; src/acquire.jl:221-228 stores them).
+ * nbins = 0 (default): the reference's (ndop, spc_eff) surface, rows in
+ * sorted-Doppler order, columns in scrambled FM-DBZP order. nbins in [1, spc]:
+ * a chip-ordered (ndop, nbins) surface, reduced on the device. chip_order
+ * holds the spc code columns in ascending code phase
+ * (mod(-tau f_code / fs, spc f_code / fs) of `_fmdbzp_sort_by_chip`, a stable
+ * order), and bin b is the maximum over columns chip_order[b spc / nbins ..
+ * (b + 1) spc / nbins) and over the R rotation slices of each. With nbins =
+ * spc and R = 1 it is the reference's chip-sorted surface itself. A 25 MHz
+ * E1-like surface is 20 GB; 8184 bins (two per chip) are 65 MB.
+ */
+int fmdbzp_plan_set_surface_layout(fmdbzp_plan* plan, int32_t nbins, const int32_t* chip_order);
+
 /* --------------------------------------------------------------------------
  * Seeded synthetic signals generated on the device: the analogue of
  * `generate_test_signal` (src/generate_test_signal.jl:32-82) for benchmark
