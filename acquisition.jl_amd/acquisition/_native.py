@@ -114,6 +114,16 @@ class SceneDesc(ctypes.Structure):
     ]
 
 
+class SecondaryRequest(ctypes.Structure):
+    _fields_ = [
+        ("code", ctypes.c_void_p),
+        ("secondary", ctypes.c_void_p),
+        ("doppler_hz", ctypes.c_double),
+        ("num_rotations", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 EXPORTED_SYMBOLS = (
     "fmdbzp_plan_create",
     "fmdbzp_acquire",
@@ -125,6 +135,7 @@ EXPORTED_SYMBOLS = (
     "fmdbzp_plan_kernel_times",
     "fmdbzp_synthesize",
     "fmdbzp_plan_set_surface_layout",
+    "fmdbzp_secondary_phase",
 )
 
 
@@ -152,6 +163,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fmdbzp_plan_set_timing.restype = ctypes.c_int
     lib.fmdbzp_plan_kernel_times.argtypes = [P, P, P, ctypes.c_int32]
     lib.fmdbzp_plan_kernel_times.restype = ctypes.c_int
+    lib.fmdbzp_secondary_phase.argtypes = [P, P, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32,
+                                           P, ctypes.c_int32, P]
+    lib.fmdbzp_secondary_phase.restype = ctypes.c_int
     lib.fmdbzp_plan_set_surface_layout.argtypes = [P, ctypes.c_int32, P]
     lib.fmdbzp_plan_set_surface_layout.restype = ctypes.c_int
     lib.fmdbzp_synthesize.argtypes = [ctypes.POINTER(SceneDesc), P, ctypes.c_int32, P]
@@ -250,6 +264,21 @@ class NativePlan:
                     ctypes.cast(stats, ctypes.c_void_p).value,
                     ctypes.cast(surf_arg, ctypes.c_void_p).value if surf_arg is not None else None)
         return q, (idx, stats, surf_arg)
+
+    def secondary_phase(self, signal_ptr: int, eltype: int, n_samples: int, if_hz: float, flags: int, items):
+        """items: (code int8 [spc], secondary int8 [L], doppler_hz) per detected PRN; returns the phases."""
+        keep = []
+        arr = (SecondaryRequest * len(items))()
+        for i, (code, sec, dop) in enumerate(items):
+            c = np.ascontiguousarray(code, dtype=np.int8)
+            q = np.ascontiguousarray(sec, dtype=np.int8)
+            keep += [c, q]
+            arr[i] = SecondaryRequest(c.ctypes.data, q.ctypes.data, float(dop), len(q), 0)
+        out = np.zeros(len(items), dtype=np.int32)
+        _check(lib().fmdbzp_secondary_phase(self.handle, ctypes.c_void_p(signal_ptr), eltype, n_samples,
+                                            float(if_hz), flags, ctypes.cast(arr, ctypes.c_void_p), len(items),
+                                            out.ctypes.data), "fmdbzp_secondary_phase")
+        return [int(v) for v in out]
 
     def set_surface_layout(self, nbins: int, chip_order=None):
         order = None if chip_order is None else np.ascontiguousarray(chip_order, dtype=np.int32)

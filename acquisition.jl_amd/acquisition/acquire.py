@@ -204,56 +204,6 @@ class _SignalView:
             self.device = False
             self.ptr = a.ctypes.data
 
-    def complex64_prefix(self, n: int) -> np.ndarray:
-        """ComplexF32 of the first n samples on the host."""
-        from .device import DeviceArray
-
-        if isinstance(self.obj, Packed2BitSignal):
-            return self.obj.unpack(n)
-        if isinstance(self.obj, DeviceArray):
-            x = self.obj.to_host(n)
-        else:
-            x = self.obj[:n]
-        if _is_torch(x):
-            x = x.cpu().numpy()
-        if self.kind in ("ci8", "ci16"):
-            return (x[:, 0].astype(np.float32) + 1j * x[:, 1].astype(np.float32)).astype(np.complex64)
-        return np.asarray(x).astype(np.complex64)
-
-
-# ---------------------------------------------------------------------------
-# secondary-code phase (host, on the raw signal)
-# ---------------------------------------------------------------------------
-def _estimate_secondary_code_phase(plan: AcquisitionPlan, prn: int, sig: np.ndarray, if_hz: float,
-                                   fs: float, code_phase: float, doppler_hz: float) -> int:
-    """`_estimate_secondary_code_phase` (src/acquire.jl:503-538): despread the
-    first L periods at the recovered (Doppler, code phase) and take the
-    strongest of the L secondary-code rotations."""
-    system = plan.system
-    L = plan.num_secondary_rotations
-    spc = plan.samples_per_code
-    code = system.gen_code(spc, prn, fs, system.get_code_frequency(), code_phase).astype(np.float32)
-    phase_step = -2.0 * math.pi * (if_hz + doppler_hz) / fs
-    dphi = np.complex64(complex(f32(math.cos(phase_step)), f32(math.sin(phase_step))))
-    zs = []
-    for k in range(L):
-        base = k * spc
-        a = phase_step * base
-        seq = np.empty(spc, dtype=np.complex64)
-        seq[0] = np.complex64(complex(f32(math.cos(a)), f32(math.sin(a))))
-        seq[1:] = dphi
-        carr = np.cumprod(seq, dtype=np.complex64)
-        zs.append(np.complex64(np.sum((sig[base:base + spc] * carr) * code, dtype=np.complex64)))
-    best_r, best = 0, f32(-1.0)
-    for r in range(L):
-        s = np.complex64(0)
-        for k in range(L):
-            s = np.complex64(s + np.complex64(f32(system.secondary_value(prn, (k + r) % L)) * zs[k]))
-        mag = f32(f32(s.real * s.real) + f32(s.imag * s.imag))
-        if mag > best:
-            best, best_r = mag, r
-    return best_r
-
 
 # ---------------------------------------------------------------------------
 # acquire_
@@ -297,21 +247,22 @@ def _extract(plan: AcquisitionPlan, st, prn: int, sigview: _SignalView, if_hz: f
             frac = parabolic_interp(dl, peak, dr)
             doppler = doppler + float(frac) * plan.doppler_spacing_hz
 
+    # the secondary-code phase of a detected PRN (src/acquire.jl:646-652) is
+    # filled in by _results_from_stats, for all such PRNs in one device call
     secondary_code_phase = None
+    needs_secondary = False
     if plan.num_secondary_rotations > 1:
         num_cells = ndop * plan.samples_per_code * plan.num_secondary_rotations
         thr = cfar_threshold(0.01, num_cells, num_noncoherent_integrations=plan.num_noncoherent_accumulations)
-        if float(ptn) > thr:
-            n = plan.num_secondary_rotations * spc
-            sig = sigview.complex64_prefix(n)
-            secondary_code_phase = _estimate_secondary_code_phase(plan, prn, sig, if_hz, fs, code_phase, doppler)
+        needs_secondary = float(ptn) > thr
 
-    return AcquisitionResults(
+    r = AcquisitionResults(
         system=system, prn=prn, sampling_frequency=fs, carrier_doppler=doppler, code_phase=code_phase,
         secondary_code_phase=secondary_code_phase, CN0=cn0, noise_power=noise, peak_to_noise_ratio=ptn,
         num_noncoherent_integrations=plan.num_noncoherent_accumulations, power_bins=power_bins,
         dopplers=plan.doppler_freqs, num_blocks=plan.num_blocks, block_size=plan.block_size,
         num_secondary_rotations=plan.num_secondary_rotations)
+    return r, needs_secondary
 
 
 def _get_result_buffer(plan: AcquisitionPlan, prn_idx: int) -> np.ndarray:
@@ -376,16 +327,28 @@ def _prepare(plan: AcquisitionPlan, signal, prns, interm_freq, store_power_bins)
 
 
 def _results_from_stats(plan, stats, prns, sv, interm_freq, subsample_interpolation, bufs):
-    results = []
+    results, pending = [], []
     for j, p in enumerate(prns):
-        r = _extract(plan, stats[j], p, sv, float(interm_freq), subsample_interpolation,
-                     bufs[j] if bufs is not None else None)
+        r, needs_secondary = _extract(plan, stats[j], p, sv, float(interm_freq), subsample_interpolation,
+                                      bufs[j] if bufs is not None else None)
+        if needs_secondary:
+            pending.append(r)
         st = stats[j]
         r.engine_stats = dict(peak=float(st.peak_power), row=int(st.peak_row), col=int(st.peak_col),
                               noise=float(st.noise_power), nbr=[float(st.code_left), float(st.code_right),
                                                                 float(st.dop_left), float(st.dop_right)],
                               noise_sum=float(st.noise_sum), noise_cells=int(st.noise_cells))
         results.append(r)
+    if pending:
+        # `_estimate_secondary_code_phase` (src/acquire.jl:503-538) on the device
+        sysm, spc, fs = plan.system, plan.samples_per_code, plan.sampling_freq
+        items = [(sysm.gen_code(spc, r.prn, fs, sysm.get_code_frequency(), r.code_phase),
+                  np.array([sysm.secondary_value(r.prn, k) for k in range(plan.num_secondary_rotations)],
+                           dtype=np.int8), r.carrier_doppler) for r in pending]
+        flags = _native.FLAG_DEVICE_SIGNAL if sv.device else 0
+        phases = plan.native.secondary_phase(sv.ptr, _native.ELTYPE[sv.kind], sv.n, float(interm_freq), flags, items)
+        for r, ph in zip(pending, phases):
+            r.secondary_code_phase = ph
     return results
 
 

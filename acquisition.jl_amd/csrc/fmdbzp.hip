@@ -703,6 +703,88 @@ int fmdbzp_plan_set_surface_layout(fmdbzp_plan* plan, int32_t nbins, const int32
     return FMDBZP_OK;
 }
 
+int fmdbzp_secondary_phase(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_t n_samples,
+                           double interm_freq_hz, uint32_t flags, const fmdbzp_secondary_request* reqs,
+                           int32_t nreq, int32_t* out_phase)
+{
+    g_err.clear();
+    if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");
+    if (nreq < 0 || (nreq > 0 && (!reqs || !out_phase || !signal)))
+        return set_err(FMDBZP_EINVAL, "null argument to fmdbzp_secondary_phase");
+    if (nreq == 0) return FMDBZP_OK;
+    if (!elsize(eltype) && eltype != FMDBZP_C2BIT) return set_err(FMDBZP_EINVAL, "unknown eltype %d", eltype);
+    std::lock_guard<std::mutex> lk(plan->mu);
+    const int spc = plan->d.samples_per_code;
+    int Lmax = 0;
+    std::vector<int8_t> codes((size_t)nreq * spc), secs((size_t)nreq * 64, 1);
+    for (int i = 0; i < nreq; i++) {
+        const fmdbzp_secondary_request& q = reqs[i];
+        if (q.num_rotations < 1 || q.num_rotations > 64 || !q.code || !q.secondary)
+            return set_err(FMDBZP_EINVAL, "request %d: 1 <= num_rotations <= 64 and code / secondary required", i);
+        for (int n = 0; n < spc; n++) {
+            if (q.code[n] != 1 && q.code[n] != -1) return set_err(FMDBZP_EINVAL, "request %d: code must be +-1", i);
+            codes[(size_t)i * spc + n] = q.code[n];
+        }
+        for (int k = 0; k < q.num_rotations; k++) {
+            if (q.secondary[k] != 1 && q.secondary[k] != -1)
+                return set_err(FMDBZP_EINVAL, "request %d: secondary code must be +-1", i);
+            secs[(size_t)i * 64 + k] = q.secondary[k];
+        }
+        Lmax = std::max(Lmax, q.num_rotations);
+    }
+    if (n_samples < (int64_t)Lmax * spc)
+        return set_err(FMDBZP_EINVAL, "signal has %lld samples, fewer than %d code periods", (long long)n_samples, Lmax);
+    HIP_TRY(hipSetDevice(plan->device));
+    if (plan->pending_out) {
+        const int rc = acquire_finish(plan);
+        if (rc) return rc;
+    }
+    const size_t nsig = (size_t)Lmax * spc;
+    const size_t sig_bytes = eltype == FMDBZP_C2BIT ? (nsig + 1) / 2 : nsig * elsize(eltype);
+    void* d_sig = nullptr;
+    int8_t *d_codes = nullptr, *d_secs = nullptr;
+    SecReqDev* d_reqs = nullptr;
+    int32_t* d_out = nullptr;
+    std::vector<SecReqDev> rq((size_t)nreq);
+    hipError_t e = hipSuccess;
+    const bool dev_sig = (flags & FMDBZP_FLAG_DEVICE_SIGNAL) != 0;
+    if (!dev_sig) {
+        e = hipMalloc(&d_sig, sig_bytes);
+        if (e == hipSuccess) e = hipMemcpy(d_sig, signal, sig_bytes, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&d_codes, codes.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&d_secs, secs.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&d_reqs, sizeof(SecReqDev) * (size_t)nreq);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_out, sizeof(int32_t) * (size_t)nreq);
+    if (e == hipSuccess) e = hipMemcpy(d_codes, codes.data(), codes.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_secs, secs.data(), secs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const double twopi = 6.283185307179586;  // Julia `2π`
+        for (int i = 0; i < nreq; i++) {
+            rq[(size_t)i].code = d_codes + (size_t)i * spc;
+            rq[(size_t)i].sec = d_secs + (size_t)i * 64;
+            rq[(size_t)i].phase_step = (-twopi * (interm_freq_hz + reqs[i].doppler_hz)) / plan->d.sampling_freq_hz;
+            rq[(size_t)i].L = reqs[i].num_rotations;
+            rq[(size_t)i].pad = 0;
+        }
+        e = hipMemcpy(d_reqs, rq.data(), sizeof(SecReqDev) * (size_t)nreq, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) {
+        k_secondary_phase<<<nreq, 64, 0, plan->stream>>>(dev_sig ? signal : d_sig, eltype, spc, d_reqs, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out_phase, d_out, sizeof(int32_t) * (size_t)nreq, hipMemcpyDeviceToHost,
+                                            plan->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(plan->stream);
+    hipFree(d_sig);
+    hipFree(d_codes);
+    hipFree(d_secs);
+    hipFree(d_reqs);
+    hipFree(d_out);
+    if (e != hipSuccess) return set_err(FMDBZP_EHIP, "fmdbzp_secondary_phase: %s", hipGetErrorString(e));
+    return FMDBZP_OK;
+}
+
 int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launches, int32_t n)
 {
     if (!plan) return set_err(FMDBZP_EHANDLE, "null plan");

@@ -1070,6 +1070,69 @@ k_export_cm(const float* __restrict__ acc, float* __restrict__ out, int ndop, in
     if (row < ndop) out[(size_t)col * ndop + row] = acc[(size_t)col * ldc + row];
 }
 
+// Secondary-code phase of a detected PRN (`_estimate_secondary_code_phase`,
+// src/acquire.jl:503-538), in the reference's order of operations: lane k
+// despreads code period k sequentially, acc += (x * carr) * code with
+// ComplexF32 products (no fused multiply-add) and carr *= dphi, the phasor
+// reset per period from an FP64 sincos; lane 0 then sums the L rotations of
+// the secondary code and keeps the first largest |s|^2. grid (requests),
+// block 64 (L <= 64).
+struct SecReqDev {
+    const int8_t* code;  // spc samples at the estimated code phase
+    const int8_t* sec;   // L secondary chips
+    double phase_step;   // -2 pi (IF + Doppler) / fs
+    int L;
+    int pad;
+};
+__global__ void __launch_bounds__(64)
+k_secondary_phase(const void* __restrict__ sig, int eltype, int spc, const SecReqDev* __restrict__ reqs,
+                  int32_t* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const SecReqDev R = reqs[blockIdx.x];
+    __shared__ float zr[64], zi[64];
+    const int k = threadIdx.x;
+    if (k < R.L) {
+        const int64_t base = (int64_t)k * spc;
+        const double a = R.phase_step * (double)base;
+        float cr = (float)cos(a), ci = (float)sin(a);
+        const float dre = (float)cos(R.phase_step), dim = (float)sin(R.phase_step);
+        float ar = 0.0f, ai = 0.0f;
+        for (int n = 0; n < spc; n++) {
+            const float2 x = load_sample(sig, eltype, base + n);
+            const float pr = x.x * cr - x.y * ci, pim = x.x * ci + x.y * cr;
+            const float c = (float)R.code[n];
+            const float qr = pr * c - pim * 0.0f, qi = pr * 0.0f + pim * c;
+            ar += qr;
+            ai += qi;
+            const float ncr = cr * dre - ci * dim, nci = cr * dim + ci * dre;
+            cr = ncr;
+            ci = nci;
+        }
+        zr[k] = ar;
+        zi[k] = ai;
+    }
+    __syncthreads();
+    if (k == 0) {
+        int best_r = 0;
+        float best = -1.0f;
+        for (int r = 0; r < R.L; r++) {
+            float sr = 0.0f, si = 0.0f;
+            for (int q = 0; q < R.L; q++) {
+                const float sv = (float)R.sec[(q + r) % R.L];
+                sr += sv * zr[q];
+                si += sv * zi[q];
+            }
+            const float mag = sr * sr + si * si;
+            if (mag > best) {
+                best = mag;
+                best_r = r;
+            }
+        }
+        out[blockIdx.x] = best_r;
+    }
+}
+
 // Chip-ordered surface export (fmdbzp_plan_set_surface_layout): grid (nbins,
 // ceil(ndop / NTHR)). Bin b holds, per sorted Doppler row, the maximum over
 // the code columns order[b spc / nbins .. (b + 1) spc / nbins) (ascending code

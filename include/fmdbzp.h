@@ -265,6 +265,31 @@ int fmdbzp_plan_kernel_times(fmdbzp_plan* plan, double* total_ms, int64_t* launc
  */
 int fmdbzp_plan_set_surface_layout(fmdbzp_plan* plan, int32_t nbins, const int32_t* chip_order);
 
+/*
+ * Secondary-code phase of detected PRNs on the device: the replacement for
+ * `_estimate_secondary_code_phase` (src/acquire.jl:503-538), called by the
+ * result extraction for each PRN whose peak-to-noise ratio passes
+ * cfar_threshold(0.01, ndop spc R) (src/acquire.jl:646-652). Each request
+ * despreads the first L code periods of `signal` at its estimated Doppler
+ * (carrier interm_freq_hz + doppler_hz at the plan's sampling rate) with
+ * `code` (spc samples of gen_code at the estimated code phase), in the
+ * reference's order of operations, and returns the rotation r in [0, L) whose
+ * sum over periods of secondary[(k + r) mod L] z_k has the largest |.|^2
+ * (first one on ties). The signal is in device memory with
+ * FMDBZP_FLAG_DEVICE_SIGNAL, else on the host (its first L spc samples are
+ * copied). Synchronous.
+ */
+typedef struct fmdbzp_secondary_request {
+    const int8_t* code;       /* spc samples, +-1 (host) */
+    const int8_t* secondary;  /* num_rotations secondary chips, +-1 (host) */
+    double doppler_hz;
+    int32_t num_rotations;    /* L, 1..64 */
+    int32_t reserved;
+} fmdbzp_secondary_request;
+int fmdbzp_secondary_phase(fmdbzp_plan* plan, const void* signal, int32_t eltype, int64_t n_samples,
+                           double interm_freq_hz, uint32_t flags, const fmdbzp_secondary_request* reqs,
+                           int32_t nreq, int32_t* out_phase);
+
 /* --------------------------------------------------------------------------
  * Seeded synthetic signals generated on the device: the analogue of
  * `generate_test_signal` (src/generate_test_signal.jl:32-82) for benchmark

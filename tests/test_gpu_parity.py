@@ -299,6 +299,42 @@ def test_e1like_rotation_boc_reduced():
     assert res[0].secondary_code_phase == sc.truth[p0]["secondary_phase"]
 
 
+def test_secondary_phase_on_device_matches_oracle():
+    """`fmdbzp_secondary_phase` against the oracle's restatement of
+    `_estimate_secondary_code_phase` (src/acquire.jl:503-538): planted NH10
+    rotations at several code phases and Dopplers, host and device signals,
+    complex64 and complex int16 samples, one batched call."""
+    from acquisition.device import DeviceArray
+    from acquisition import _native
+
+    sysm = GPSL5I()
+    fs = 12e6
+    plan = plan_acquire(sysm, fs, [1], num_coherently_integrated_code_periods=10)
+    try:
+        cases = [(0, 0.0, 0.0), (3, 2000.0, 0.0), (7, 5115.0, 1300.0), (9, 9000.0, -2100.0), (5, 77.5, 640.0)]
+        sec = sysm.secondary_code(1)
+        for r0, cp, dop in cases:
+            sig = l5i_clean(fs=fs, r0=r0, cp=cp)
+            n = np.arange(sig.shape[0])
+            sig = (sig * np.exp(2j * np.pi * dop * n / fs) * 900.0).astype(np.complex64)
+            ci16 = np.stack([np.rint(sig.real), np.rint(sig.imag)], axis=1).astype(np.int16)
+            code = sysm.gen_code(12000, 1, fs, sysm.code_frequency, cp)
+            want = oracle.secondary_phase(sig, 12000, 10, fs, 0.0, dop, code, sec)
+            assert want == r0
+            got_host = plan.native.secondary_phase(sig.ctypes.data, _native.ELTYPE["cf32"], sig.shape[0], 0.0, 0,
+                                                   [(code, sec, dop)])
+            dev = DeviceArray(ci16)
+            try:
+                got_dev = plan.native.secondary_phase(dev.ptr, _native.ELTYPE["ci16"], ci16.shape[0], 0.0,
+                                                      _native.FLAG_DEVICE_SIGNAL, [(code, sec, dop)] * 2)
+            finally:
+                dev.free()
+            assert got_host == [r0] and got_dev == [oracle.secondary_phase(ci16, 12000, 10, fs, 0.0, dop, code,
+                                                                            sec)] * 2
+    finally:
+        plan.close()
+
+
 def test_correlation_f2500():
     """GPS L1 C/A 5 MHz, +-1 kHz, N_coh 2, N_nc 2: nb 4, bs 1250, F 2500, the
     block-correlation size of the 25 MHz E1-like search (compile-time
