@@ -178,6 +178,8 @@ struct fmdbzp_plan {
     float* d_export = nullptr;  // one staging surface per lane (ndop x surface columns)
     int surf_bins = 0;          // 0: scrambled (ndop, spc_eff) surfaces; > 0: chip-ordered (ndop, surf_bins)
     int32_t* d_chip_order = nullptr;  // [spc] code columns in ascending code phase
+    char* d_sec = nullptr;            // secondary-phase requests: codes, chips, descriptors, results
+    size_t sec_cap = 0;
     int batch = 1;
     int LB = 1, bstride = 1, sst = 1;
     bool simple_kernel = false;  // column stage runs k_column_simple
@@ -365,6 +367,7 @@ void free_plan(fmdbzp_plan* P)
     hipFree(P->d_stats);
     hipFree(P->d_export);
     hipFree(P->d_chip_order);
+    hipFree(P->d_sec);
     hipHostFree(P->h_req);
     hipHostFree(P->h_stats);
     hipHostFree(P->h_signal);
@@ -741,46 +744,52 @@ int fmdbzp_secondary_phase(fmdbzp_plan* plan, const void* signal, int32_t eltype
     }
     const size_t nsig = (size_t)Lmax * spc;
     const size_t sig_bytes = eltype == FMDBZP_C2BIT ? (nsig + 1) / 2 : nsig * elsize(eltype);
-    void* d_sig = nullptr;
-    int8_t *d_codes = nullptr, *d_secs = nullptr;
-    SecReqDev* d_reqs = nullptr;
-    int32_t* d_out = nullptr;
-    std::vector<SecReqDev> rq((size_t)nreq);
-    hipError_t e = hipSuccess;
     const bool dev_sig = (flags & FMDBZP_FLAG_DEVICE_SIGNAL) != 0;
-    if (!dev_sig) {
-        e = hipMalloc(&d_sig, sig_bytes);
-        if (e == hipSuccess) e = hipMemcpy(d_sig, signal, sig_bytes, hipMemcpyHostToDevice);
+    if (!dev_sig && plan->signal_cap < sig_bytes) {  // host samples go through the plan's signal buffer
+        hipFree(plan->d_signal);
+        plan->d_signal = nullptr;
+        plan->signal_cap = 0;
+        HIP_TRY(hipMalloc(&plan->d_signal, sig_bytes));
+        plan->signal_cap = sig_bytes;
     }
-    if (e == hipSuccess) e = hipMalloc((void**)&d_codes, codes.size());
-    if (e == hipSuccess) e = hipMalloc((void**)&d_secs, secs.size());
-    if (e == hipSuccess) e = hipMalloc((void**)&d_reqs, sizeof(SecReqDev) * (size_t)nreq);
-    if (e == hipSuccess) e = hipMalloc((void**)&d_out, sizeof(int32_t) * (size_t)nreq);
+    // plan-owned request buffer: codes | secondary chips | descriptors | results
+    const size_t o_sec = ((size_t)nreq * spc + 255) / 256 * 256;
+    const size_t o_req = o_sec + ((size_t)nreq * 64 + 255) / 256 * 256;
+    const size_t o_out = o_req + (sizeof(SecReqDev) * (size_t)nreq + 255) / 256 * 256;
+    const size_t need = o_out + sizeof(int32_t) * (size_t)nreq;
+    if (plan->sec_cap < need) {
+        hipFree(plan->d_sec);
+        plan->d_sec = nullptr;
+        plan->sec_cap = 0;
+        HIP_TRY(hipMalloc((void**)&plan->d_sec, need));
+        plan->sec_cap = need;
+    }
+    int8_t* d_codes = (int8_t*)plan->d_sec;
+    int8_t* d_secs = (int8_t*)(plan->d_sec + o_sec);
+    SecReqDev* d_reqs = (SecReqDev*)(plan->d_sec + o_req);
+    int32_t* d_out = (int32_t*)(plan->d_sec + o_out);
+    std::vector<SecReqDev> rq((size_t)nreq);
+    const double twopi = 6.283185307179586;  // Julia `2π`
+    for (int i = 0; i < nreq; i++) {
+        rq[(size_t)i].code = d_codes + (size_t)i * spc;
+        rq[(size_t)i].sec = d_secs + (size_t)i * 64;
+        rq[(size_t)i].phase_step = (-twopi * (interm_freq_hz + reqs[i].doppler_hz)) / plan->d.sampling_freq_hz;
+        rq[(size_t)i].L = reqs[i].num_rotations;
+        rq[(size_t)i].pad = 0;
+    }
+    hipError_t e = hipSuccess;
+    if (!dev_sig) e = hipMemcpy(plan->d_signal, signal, sig_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_codes, codes.data(), codes.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_secs, secs.data(), secs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_reqs, rq.data(), sizeof(SecReqDev) * (size_t)nreq, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        const double twopi = 6.283185307179586;  // Julia `2π`
-        for (int i = 0; i < nreq; i++) {
-            rq[(size_t)i].code = d_codes + (size_t)i * spc;
-            rq[(size_t)i].sec = d_secs + (size_t)i * 64;
-            rq[(size_t)i].phase_step = (-twopi * (interm_freq_hz + reqs[i].doppler_hz)) / plan->d.sampling_freq_hz;
-            rq[(size_t)i].L = reqs[i].num_rotations;
-            rq[(size_t)i].pad = 0;
-        }
-        e = hipMemcpy(d_reqs, rq.data(), sizeof(SecReqDev) * (size_t)nreq, hipMemcpyHostToDevice);
-    }
-    if (e == hipSuccess) {
-        k_secondary_phase<<<nreq, 64, 0, plan->stream>>>(dev_sig ? signal : d_sig, eltype, spc, d_reqs, d_out);
+        k_secondary_phase<<<nreq, 64, 0, plan->stream>>>(dev_sig ? signal : plan->d_signal, eltype, spc, d_reqs,
+                                                         d_out);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out_phase, d_out, sizeof(int32_t) * (size_t)nreq, hipMemcpyDeviceToHost,
                                             plan->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(plan->stream);
-    hipFree(d_sig);
-    hipFree(d_codes);
-    hipFree(d_secs);
-    hipFree(d_reqs);
-    hipFree(d_out);
     if (e != hipSuccess) return set_err(FMDBZP_EHIP, "fmdbzp_secondary_phase: %s", hipGetErrorString(e));
     return FMDBZP_OK;
 }
