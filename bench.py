@@ -322,15 +322,21 @@ def main():
     det = sorted(r.prn for r in res if is_detected(r))
     log(f"warmup done; detected {det} (present {sc.present})")
 
-    plan.native.set_timing(True)
+    # value: the steps without per-kernel events; then the same steps again with
+    # HIP events around every launch for the per-kernel times (roofline)
     t0 = time.perf_counter()
     for i in range(args.steps):
         res = acquire_(plan, dev, prns, interm_freq=sc.interm_freq)
     t1 = time.perf_counter()
+    dt = (t1 - t0) / args.steps
+    plan.native.set_timing(True)
+    t2 = time.perf_counter()
+    for i in range(args.steps):
+        acquire_(plan, dev, prns, interm_freq=sc.interm_freq)
+    dt_ev = (time.perf_counter() - t2) / args.steps
     kt = plan.native.kernel_times()
     plan.native.set_timing(False)
-    dt = (t1 - t0) / args.steps
-    log(f"timed {args.steps} steps: {dt * 1e3:.3f} ms/step; kernel ms totals "
+    log(f"timed {args.steps} steps: {dt * 1e3:.3f} ms/step ({dt_ev * 1e3:.3f} with kernel events); kernel ms totals "
         + ", ".join(f"{k} {v[0]:.2f}/{v[1]}" for k, v in kt.items() if v[1]))
 
     # the same steps with the samples in host memory (pageable numpy array:
