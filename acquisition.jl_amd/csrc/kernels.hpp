@@ -197,10 +197,17 @@ struct LagOut {
     __device__ __forceinline__ void operator()(int t, int i, float2 v) const
     {
         if (i < bs && t < nt) {
-            if constexpr (sizeof(TileT) == sizeof(float2))
-                out[t * tstride + i] = make_float2(v.x * inv_f, v.y * inv_f);
-            else
-                out[t * tstride + i] = __floats2half2_rn((v.x * inv_f) * hscale, (v.y * inv_f) * hscale);
+            // the tile is written once and read by the next kernel: nontemporal stores
+            if constexpr (sizeof(TileT) == sizeof(float2)) {
+                typedef float f2v __attribute__((ext_vector_type(2)));
+                f2v u;
+                u.x = v.x * inv_f;
+                u.y = v.y * inv_f;
+                __builtin_nontemporal_store(u, (f2v*)&out[t * tstride + i]);
+            } else {
+                const __half2 h = __floats2half2_rn((v.x * inv_f) * hscale, (v.y * inv_f) * hscale);
+                __builtin_nontemporal_store(*(const uint32_t*)&h, (uint32_t*)&out[t * tstride + i]);
+            }
         }
     }
 };
@@ -376,12 +383,18 @@ k_column_c160(ColumnArgs A, float hinv)
                     const __half2* Th = (const __half2*)A.tile + (((size_t)p * A.nb + r) * N) * A.tp + lag0;
 #pragma unroll
                     for (int m = 0; m < R0; m++) {
-                        const float2 u = __half22float2(Th[(size_t)(j + M0 * m) * A.tp + t]);
+                        const uint32_t raw = __builtin_nontemporal_load((const uint32_t*)&Th[(size_t)(j + M0 * m) * A.tp + t]);
+                        const float2 u = __half22float2(*(const __half2*)&raw);
                         v[m] = make_float2(u.x * hinv, u.y * hinv);
                     }
                 } else {
+                    // the tile is read once: nontemporal, so it does not displace accumulator lines
 #pragma unroll
-                    for (int m = 0; m < R0; m++) v[m] = T[(size_t)(j + M0 * m) * A.tp + t];
+                    for (int m = 0; m < R0; m++) {
+                        typedef float f2v __attribute__((ext_vector_type(2)));
+                        const f2v u = __builtin_nontemporal_load((const f2v*)&T[(size_t)(j + M0 * m) * A.tp + t]);
+                        v[m] = make_float2(u.x, u.y);
+                    }
                 }
                 bfly10<false>(v);
 #pragma unroll
@@ -728,6 +741,7 @@ k_column_rot2000(ColumnArgs A)
     for (int k = tid; k < N; k += FMDBZP_NTHR) {
         int src = k + eff;
         if (src >= N) src -= N;
+        // neighbouring workgroups (the next lags) read the same tile lines: a plain, cached load
         xr[k] = T[(size_t)src * A.tp];
     }
     if (tid < A.R) {
