@@ -293,6 +293,17 @@ __device__ __forceinline__ void acc_cell(const ColumnArgs& A, int p, int row, in
     else *dst = *dst + v;
 }
 
+// Tile values are read once per step. For the L1 C/A 10 ms searches (tiles of
+// hundreds of MB per step) nontemporal loads keep them from displacing the
+// accumulator lines in the caches; small tiles stay cached between kernels,
+// and the other column kernels load them normally.
+__device__ __forceinline__ float2 load_tile_nt(const float2* p)
+{
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v u = __builtin_nontemporal_load((const f2v*)p);
+    return make_float2(u.x, u.y);
+}
+
 struct TileColIn {
     const float2* T;  // tile of (p, r), offset to lag0
     int bs;
@@ -390,11 +401,7 @@ k_column_c160(ColumnArgs A, float hinv)
                 } else {
                     // the tile is read once: nontemporal, so it does not displace accumulator lines
 #pragma unroll
-                    for (int m = 0; m < R0; m++) {
-                        typedef float f2v __attribute__((ext_vector_type(2)));
-                        const f2v u = __builtin_nontemporal_load((const f2v*)&T[(size_t)(j + M0 * m) * A.tp + t]);
-                        v[m] = make_float2(u.x, u.y);
-                    }
+                    for (int m = 0; m < R0; m++) v[m] = load_tile_nt(&T[(size_t)(j + M0 * m) * A.tp + t]);
                 }
                 bfly10<false>(v);
 #pragma unroll
