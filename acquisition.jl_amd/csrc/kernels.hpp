@@ -1119,16 +1119,32 @@ k_secondary_phase(const void* __restrict__ sig, int eltype, int spc, const SecRe
         float cr = (float)cos(a), ci = (float)sin(a);
         const float dre = (float)cos(R.phase_step), dim = (float)sin(R.phase_step);
         float ar = 0.0f, ai = 0.0f;
-        for (int n = 0; n < spc; n++) {
-            const float2 x = load_sample(sig, eltype, base + n);
-            const float pr = x.x * cr - x.y * ci, pim = x.x * ci + x.y * cr;
-            const float c = (float)R.code[n];
-            const float qr = pr * c - pim * 0.0f, qi = pr * 0.0f + pim * c;
-            ar += qr;
-            ai += qi;
-            const float ncr = cr * dre - ci * dim, nci = cr * dim + ci * dre;
-            cr = ncr;
-            ci = nci;
+        // samples and code chips are loaded 16 at a time ahead of the
+        // recurrence, which then runs over them in order
+        constexpr int B = 16;
+        for (int n0 = 0; n0 < spc; n0 += B) {
+            float2 xs[B];
+            float cs[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int n = min(n0 + u, spc - 1);
+                xs[u] = load_sample(sig, eltype, base + n);
+                cs[u] = (float)R.code[n];
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                if (n0 + u < spc) {
+                    const float2 x = xs[u];
+                    const float pr = x.x * cr - x.y * ci, pim = x.x * ci + x.y * cr;
+                    const float c = cs[u];
+                    const float qr = pr * c - pim * 0.0f, qi = pr * 0.0f + pim * c;
+                    ar += qr;
+                    ai += qi;
+                    const float ncr = cr * dre - ci * dim, nci = cr * dim + ci * dre;
+                    cr = ncr;
+                    ci = nci;
+                }
+            }
         }
         zr[k] = ar;
         zi[k] = ai;
