@@ -1,0 +1,518 @@
+"""Sampling-frequency recommender with a cost model for this engine.
+
+`recommend_sampling_freqs` sweeps candidate sampling rates, derives the
+FM-DBZP geometry `plan_acquire` would pick at each, and ranks the candidates
+(src/recommend_sampling_freq.jl:195-296). By default a candidate's cost is
+the reference's FFT-FLOP estimate (`_fft_cost`, :118-125). With
+`cost_model=GpuCostModel(...)` the cost is instead the predicted time of this
+engine for one PRN through one noncoherent step, in microseconds. The
+prediction comes from per-kernel coefficients fitted to kernel times that the
+engine measures on the device (`GpuCostModel.calibrate`). With
+`measure_top=K` the K best candidates are then timed on the device and
+re-ranked by the measured time.
+
+The SDR clock-plan interface (`AbstractSDRClockPlan`, `AD9361ClockPlan`,
+`is_valid_sample_rate`, `sample_rate_range`) follows src/sdr_clock_plans.jl.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .plan import ArgumentError, nextprod
+from .systems import GNSSSystem
+
+__all__ = [
+    "SamplingFreqRecommendation", "recommend_sampling_freqs", "format_recommendations",
+    "AbstractSDRClockPlan", "AD9361ClockPlan", "is_valid_sample_rate", "sample_rate_range",
+    "GpuCostModel",
+]
+
+
+# ---------------------------------------------------------------------------
+# SDR clock plans (src/sdr_clock_plans.jl)
+# ---------------------------------------------------------------------------
+class AbstractSDRClockPlan:
+    """Hardware constraints on reachable sampling rates
+    (src/sdr_clock_plans.jl:8-16). Subclasses implement
+    `is_valid_sample_rate` and may narrow `sample_rate_range`, whose default is
+    (0, inf) (:38)."""
+
+    def is_valid_sample_rate(self, fs_hz: float) -> bool:
+        raise NotImplementedError
+
+    def sample_rate_range(self) -> Tuple[float, float]:
+        return (0.0, math.inf)
+
+
+def is_valid_sample_rate(plan: AbstractSDRClockPlan, fs_hz: float) -> bool:
+    return plan.is_valid_sample_rate(fs_hz)
+
+
+def sample_rate_range(plan: AbstractSDRClockPlan) -> Tuple[float, float]:
+    return plan.sample_rate_range()
+
+
+@dataclass
+class AD9361ClockPlan(AbstractSDRClockPlan):
+    """AD9361 RFIC sample-rate rule (src/sdr_clock_plans.jl:40-110): `fs` is
+    reachable when some divider d makes adc_clk = fs·d fall in
+    [adc_clk_min, adc_clk_max] and adc_clk times some power-of-two BBPLL
+    divider in [bbpll_div_min, bbpll_div_max] falls in [bbpll_min, bbpll_max]."""
+    ref_clk: float = 38.4e6
+    adc_clk_min: float = 25e6
+    adc_clk_max: float = 640e6
+    fs_min: float = 0.55e6
+    fs_max: float = 61.44e6
+    bbpll_min: float = 715e6
+    bbpll_max: float = 1430e6
+    bbpll_div_min: int = 2
+    bbpll_div_max: int = 64
+    dividers: List[int] = field(default_factory=lambda: [12, 8, 6, 4, 3, 2, 1])
+
+    def sample_rate_range(self) -> Tuple[float, float]:
+        return (self.fs_min, self.fs_max)
+
+    def _bbpll_reachable(self, adc_clk: float) -> bool:
+        div = self.bbpll_div_min
+        while div <= self.bbpll_div_max:
+            if self.bbpll_min <= adc_clk * div <= self.bbpll_max:
+                return True
+            div *= 2
+        return False
+
+    def is_valid_sample_rate(self, fs_hz: float) -> bool:
+        if not (self.fs_min <= fs_hz <= self.fs_max):
+            return False
+        for d in self.dividers:
+            adc = fs_hz * d
+            if self.adc_clk_min <= adc <= self.adc_clk_max and self._bbpll_reachable(adc):
+                return True
+        return False
+
+    def __str__(self):
+        return (f"AD9361ClockPlan(fs ∈ [{self.fs_min / 1e6}, {self.fs_max / 1e6}] MHz, "
+                f"ADC ∈ [{self.adc_clk_min / 1e6}, {self.adc_clk_max / 1e6}] MHz, dividers={self.dividers})")
+
+
+# ---------------------------------------------------------------------------
+# Integer helpers (src/recommend_sampling_freq.jl:43-104)
+# ---------------------------------------------------------------------------
+def _prime_factors(n: int) -> List[int]:
+    facs: List[int] = []
+    if n <= 1:
+        return facs
+    p = 2
+    while p * p <= n:
+        while n % p == 0:
+            facs.append(p)
+            n //= p
+        p += 1
+    if n > 1:
+        facs.append(n)
+    return facs
+
+
+_SUPERSCRIPT = "⁰¹²³⁴⁵⁶⁷⁸⁹"
+
+
+def _factor_string(n: int) -> str:
+    """n as prime powers, e.g. 744 -> "2³·3·31"."""
+    if n <= 1:
+        return str(n)
+    parts = []
+    facs = _prime_factors(n)
+    i = 0
+    while i < len(facs):
+        j = i
+        while j < len(facs) and facs[j] == facs[i]:
+            j += 1
+        k = j - i
+        parts.append(str(facs[i]) + ("" if k == 1 else "".join(_SUPERSCRIPT[int(d)] for d in str(k))))
+        i = j
+    return "·".join(parts)
+
+
+def _largest_prime_factor(n: int) -> int:
+    facs = _prime_factors(n)
+    return facs[-1] if facs else 1
+
+
+def _smallest_valid_num_blocks(spc: int, min_num_blocks: int) -> Optional[int]:
+    for d in range(min_num_blocks, spc + 1):
+        if spc % d == 0:
+            return d
+    return None
+
+
+def _fmt_hz(x: float) -> str:
+    r = round(x, 1)
+    return str(int(r)) if float(r).is_integer() else str(r)
+
+
+def _doppler_grid_string(coverage_hz: float, ndop: int) -> str:
+    """The grid plan_acquire builds: -cov/2 : cov/ndop : -cov/2 + (ndop-1)·cov/ndop
+    (src/recommend_sampling_freq.jl:106-116)."""
+    spacing = coverage_hz / ndop
+    fmin = -coverage_hz / 2
+    fmax = fmin + (ndop - 1) * spacing
+    return f"{_fmt_hz(fmin)} : {_fmt_hz(spacing)} : {_fmt_hz(fmax)} Hz"
+
+
+def _fft_cost(spc: int, nb: int, ndop: int, inner: int) -> float:
+    """Reference cost: 2·nb·F·log2 F (forward + inverse inner FFTs) plus
+    spc·ndop·log2 ndop (column FFTs) (src/recommend_sampling_freq.jl:118-125)."""
+    return 2.0 * nb * inner * math.log2(inner) + float(spc) * ndop * math.log2(ndop)
+
+
+@dataclass(frozen=True)
+class SamplingFreqRecommendation:
+    """One candidate (src/recommend_sampling_freq.jl:4-41). `cost` is the
+    reference FFT estimate, or microseconds per PRN and noncoherent step under
+    a `GpuCostModel`."""
+    sampling_freq: float
+    samples_per_code: int
+    num_blocks: int
+    block_size: int
+    inner_fft_size: int
+    num_doppler_bins: int
+    doppler_coverage: float
+    inner_max_prime: int
+    num_doppler_bins_max_prime: int
+    cost: float
+
+    def __str__(self):
+        r = self
+        return (f"SamplingFreqRecommendation(fs={r.sampling_freq} Hz, samples_per_code={r.samples_per_code} "
+                f"({_factor_string(r.samples_per_code)}), num_blocks={r.num_blocks}, block_size={r.block_size}, "
+                f"inner_fft={r.inner_fft_size} ({_factor_string(r.inner_fft_size)}), "
+                f"num_doppler_bins={r.num_doppler_bins} ({_factor_string(r.num_doppler_bins)}), "
+                f"doppler_grid={_doppler_grid_string(r.doppler_coverage, r.num_doppler_bins)}, "
+                f"cost≈{float(f'{r.cost:.4g}')})")
+
+
+_COLUMNS = ("Sampling freq (MHz)", "Samples per code", "Num blocks", "Block size", "Inner FFT size (factors)",
+            "Num Doppler bins (factors)", "Doppler grid", "Estimated cost")
+
+
+def format_recommendations(rs: Sequence[SamplingFreqRecommendation]) -> str:
+    """Full table of candidates, every column and row
+    (src/recommend_sampling_freq.jl:321-364)."""
+    if not rs:
+        return "0-element list of SamplingFreqRecommendation (no candidates)"
+    rows = [[str(round(r.sampling_freq / 1e6, 6)), str(r.samples_per_code), str(r.num_blocks), str(r.block_size),
+             f"{r.inner_fft_size} ({_factor_string(r.inner_fft_size)})",
+             f"{r.num_doppler_bins} ({_factor_string(r.num_doppler_bins)})",
+             _doppler_grid_string(r.doppler_coverage, r.num_doppler_bins), f"{r.cost:.4g}"] for r in rs]
+    w = [max(len(_COLUMNS[c]), *(len(row[c]) for row in rows)) for c in range(len(_COLUMNS))]
+    line = lambda cells: "│ " + " │ ".join(s.rjust(w[i]) for i, s in enumerate(cells)) + " │"
+    sep = "├─" + "─┼─".join("─" * x for x in w) + "─┤"
+    return "\n".join([line(_COLUMNS), sep] + [line(row) for row in rows])
+
+
+# ---------------------------------------------------------------------------
+# Cost model of this engine
+# ---------------------------------------------------------------------------
+def _corr_variant(F: int) -> str:
+    """The block-correlation kernel the engine runs for transform size F:
+    compile-time transforms for 2048 and 2500, the runtime-radix one otherwise."""
+    return {2048: "f2048", 2500: "f2500"}.get(F, "generic")
+
+
+def _column_variant(ndop: int, bs: int) -> str:
+    """The simple-path column kernel: compile-time 160-point transform for
+    ndop 160 (and bs >= 32), the runtime-radix one otherwise."""
+    return "c160" if ndop == 160 and bs >= 32 else "generic"
+
+
+def _engine_radices(n: int, radix16: bool) -> List[int]:
+    """Radix sequence the engine plans for an n-point transform: 16s (when
+    allowed), 8s, 4, 2, 5s, 3s, then larger primes (make_fft_plan in
+    csrc/fmdbzp.hip)."""
+    out, m = [], n
+    for r in ((16,) if radix16 else ()) + (8, 4, 2, 5, 3):
+        while m % r == 0:
+            out.append(r)
+            m //= r
+    f = 7
+    while m > 1:
+        while m % f == 0:
+            out.append(f)
+            m //= f
+        f += 2
+    return out
+
+
+# Real float operations per point of one Stockham stage (butterfly plus its
+# twiddle multiplies) for the engine's unrolled radices; any other prime R runs
+# a direct R-point DFT with two complex multiplies and an add per term, about
+# 14·R per point.
+_STAGE_FLOPS = {2: 5.0, 3: 8.7, 4: 8.5, 5: 12.0, 8: 12.25, 16: 16.1}
+
+
+def _transform_work(n: int, radix16: bool) -> float:
+    """Float operations of one n-point transform as the engine plans it."""
+    return n * sum(_STAGE_FLOPS.get(r, 14.0 * r) for r in _engine_radices(n, radix16))
+
+
+def _corr_work(nb: int, F: int, ndop: int) -> float:
+    """ndop·nb inverse transforms of F points per PRN-step."""
+    return ndop * nb * _transform_work(F, True)
+
+
+def _column_work(spc: int, ndop: int) -> Tuple[float, float]:
+    """(cells, transform work) per PRN-step: bytes scale with the ndop·spc
+    cells (tile read, accumulator read-modify-write), transform work with spc
+    transforms of ndop points."""
+    return float(ndop) * spc, spc * _transform_work(ndop, False) if ndop > 1 else 0.0
+
+
+# Coefficients measured on one MI355X by GpuCostModel.calibrate() with its
+# default geometry set (GPS L1 C/A rates 4-20 MHz, N_coh 1-10, 8 and 32 PRNs):
+# per launch, corr = a0 + nprn·work·a1 and column = b0 + nprn·(cells·b1 +
+# work·b2), in seconds, with the work terms of _corr_work and _column_work.
+MI355X_COEFFS: Dict[str, Dict[str, Tuple[float, ...]]] = {
+    "corr": {"f2048": (1.950e-05, 3.025e-14), "f2500": (5.553e-05, 5.043e-14), "generic": (3.727e-05, 7.759e-14)},
+    "column": {"c160": (2.385e-06, 2.884e-12, 0.0), "generic": (6.260e-06, 5.109e-12, 0.0)},
+}
+
+
+@dataclass
+class GpuCostModel:
+    """Predicted engine time of one PRN through one noncoherent step.
+
+    Per kernel launch over `nprn` PRNs:
+      block correlation  t = a0 + nprn · ndop·nb·W(F) · a1
+      column stage       t = b0 + nprn · (ndop·spc · b1 + spc·W(ndop) · b2)
+    where W(n) counts the float operations of one n-point transform over the
+    radix stages the engine plans for it (a direct DFT for primes above 5).
+    with (a0, a1) per correlation variant and (b0, b1, b2) per column variant,
+    all in seconds. The variant is the kernel the engine selects for the
+    geometry. `predict_step` returns (t_corr + t_column) / nprn.
+    """
+    corr: Dict[str, Tuple[float, float]]
+    column: Dict[str, Tuple[float, float, float]]
+    nprn: int = 32
+    device: str = ""
+
+    @classmethod
+    def mi355x(cls, nprn: int = 32) -> "GpuCostModel":
+        if not MI355X_COEFFS["corr"]:
+            raise RuntimeError("no MI355X coefficients recorded; use GpuCostModel.calibrate()")
+        return cls({k: tuple(v) for k, v in MI355X_COEFFS["corr"].items()},
+                   {k: tuple(v) for k, v in MI355X_COEFFS["column"].items()}, nprn, "MI355X")
+
+    def _coef(self, table: dict, key: str):
+        if key in table:
+            return table[key]
+        if "generic" in table:
+            return table["generic"]
+        raise KeyError(f"cost model has no coefficients for kernel variant {key!r}")
+
+    def predict_launches(self, spc: int, nb: int, F: int, ndop: int, nprn: Optional[int] = None) -> Tuple[float, float]:
+        """(correlation, column) seconds per launch over `nprn` PRNs."""
+        n = self.nprn if nprn is None else nprn
+        a0, a1 = self._coef(self.corr, _corr_variant(F))
+        b0, b1, b2 = self._coef(self.column, _column_variant(ndop, spc // nb))
+        cells, work = _column_work(spc, ndop)
+        return a0 + n * _corr_work(nb, F, ndop) * a1, b0 + n * (cells * b1 + work * b2)
+
+    def predict_step(self, spc: int, nb: int, F: int, ndop: int, nprn: Optional[int] = None) -> float:
+        """Seconds per PRN and noncoherent step."""
+        n = self.nprn if nprn is None else nprn
+        tc, tk = self.predict_launches(spc, nb, F, ndop, n)
+        return (tc + tk) / n
+
+    def cost(self, rec: SamplingFreqRecommendation) -> float:
+        """Microseconds per PRN and noncoherent step."""
+        return 1e6 * self.predict_step(rec.samples_per_code, rec.num_blocks, rec.inner_fft_size,
+                                       rec.num_doppler_bins)
+
+    # -- calibration on the device -------------------------------------------------
+    DEFAULT_GEOMETRIES = ((4e6, 1), (5e6, 1), (5e6, 4), (8e6, 2), (10e6, 1), (10e6, 5), (12e6, 2),
+                          (16.368e6, 1), (16.368e6, 4), (16.368e6, 10), (20e6, 1), (20e6, 10))
+
+    @staticmethod
+    def measure(system: GNSSSystem, sampling_freq: float, num_coherently_integrated_code_periods: int,
+                nprn: int, *, min_doppler_coverage: float = 7000.0, repeats: int = 3, seed: int = 1) -> dict:
+        """Run the engine at one geometry on the device (noise input, N_nc 2,
+        `nprn` PRNs) and return the geometry with the mean correlation and
+        column kernel time per launch (seconds), from the engine's HIP-event
+        timing."""
+        from .acquire import acquire_
+        from .plan import plan_acquire
+
+        N = num_coherently_integrated_code_periods
+        prns = list(range(1, nprn + 1))
+        plan = plan_acquire(system, sampling_freq, prns, min_doppler_coverage=min_doppler_coverage,
+                            num_coherently_integrated_code_periods=N, num_noncoherent_accumulations=2)
+        try:
+            spc, nb, F, ndop = plan.samples_per_code, plan.num_blocks, plan.fft_size, plan.num_doppler_bins
+            n = 2 * N * spc + spc
+            rng = np.random.default_rng(seed)
+            sig = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+            acquire_(plan, sig, prns, interm_freq=0.0)  # warm-up
+            plan.native.set_timing(True)
+            for _ in range(repeats):
+                acquire_(plan, sig, prns, interm_freq=0.0)
+            kt = plan.native.kernel_times()
+            plan.native.set_timing(False)
+        finally:
+            plan.close()
+        (mc, nc), (mk, nk) = kt["corr_tile"], kt["column"]
+        if nc <= 0 or nk <= 0:
+            raise RuntimeError("engine recorded no correlation or column launches")
+        return dict(fs=sampling_freq, N=N, nprn=nprn, spc=spc, nb=nb, F=F, ndop=ndop,
+                    corr_s=1e-3 * mc / nc, column_s=1e-3 * mk / nk)
+
+    @classmethod
+    def fit(cls, points: Sequence[dict], nprn: int = 32, device: str = "") -> "GpuCostModel":
+        """Nonnegative least-squares fit of the per-variant coefficients to
+        measured launches, weighted by 1/time (relative error)."""
+        from scipy.optimize import nnls
+
+        corr_rows: Dict[str, list] = {}
+        col_rows: Dict[str, list] = {}
+        for p in points:
+            corr_rows.setdefault(_corr_variant(p["F"]), []).append(
+                ([1.0, p["nprn"] * _corr_work(p["nb"], p["F"], p["ndop"])], p["corr_s"]))
+            cells, work = _column_work(p["spc"], p["ndop"])
+            col_rows.setdefault(_column_variant(p["ndop"], p["spc"] // p["nb"]), []).append(
+                ([1.0, p["nprn"] * cells, p["nprn"] * work], p["column_s"]))
+
+        def solve(rows, ncoef):
+            A = np.array([r[0] for r in rows], dtype=np.float64)
+            y = np.array([r[1] for r in rows], dtype=np.float64)
+            scale = A.max(axis=0)
+            scale[scale == 0] = 1.0
+            w = 1.0 / y
+            x, _ = nnls((A / scale) * w[:, None], y * w)
+            return tuple(float(v) for v in x / scale)[:ncoef]
+
+        corr = {k: solve(v, 2) for k, v in corr_rows.items()}
+        column = {k: solve(v, 3) for k, v in col_rows.items()}
+        return cls(corr, column, nprn, device)
+
+    @classmethod
+    def calibrate(cls, system: Optional[GNSSSystem] = None, geometries=None, nprns=(8, 32),
+                  repeats: int = 3) -> Tuple["GpuCostModel", List[dict]]:
+        """Measure the default geometry set (sampling rate, N_coh) at each PRN
+        count in `nprns` on the current device and fit the model. Returns the
+        model and the measured points."""
+        from .systems import GPSL1CA
+
+        system = GPSL1CA() if system is None else system
+        points = [cls.measure(system, fs, N, n, repeats=repeats)
+                  for fs, N in (geometries or cls.DEFAULT_GEOMETRIES) for n in nprns]
+        return cls.fit(points, nprn=max(nprns), device="calibrated"), points
+
+
+# ---------------------------------------------------------------------------
+# The recommender (src/recommend_sampling_freq.jl:195-315)
+# ---------------------------------------------------------------------------
+def recommend_sampling_freqs(code_length_or_system, code_freq: Optional[float] = None, *,
+                             fs_min: Optional[float] = None, fs_max: Optional[float] = None,
+                             min_doppler_coverage: float = 7000.0,
+                             num_coherently_integrated_code_periods: int = 1,
+                             num_alternatives: int = 5, max_prime: int = 7, sort_by: str = "cost",
+                             fs_step: float = 1000.0,
+                             sdr_clock_plan: Optional[AbstractSDRClockPlan] = None,
+                             cost_model: Optional[GpuCostModel] = None,
+                             measure_top: int = 0) -> List[SamplingFreqRecommendation]:
+    """Sweep [fs_min, fs_max] in fs_step increments and return the
+    `num_alternatives` best candidates whose column transform length
+    (num_doppler_bins) has no prime factor above `max_prime`.
+
+    Called as `recommend_sampling_freqs(code_length, code_freq, ...)` or
+    `recommend_sampling_freqs(system, ...)` (code length and frequency from
+    the system, src/recommend_sampling_freq.jl:298-315). fs_min defaults to
+    code_freq and fs_max to 1.5·fs_min. Candidates with the same
+    samples_per_code keep the first rate. `sort_by` "cost" ranks by
+    (cost, column max prime), "smoothness" by (column max prime, cost).
+    `cost_model` replaces the reference FFT estimate by this engine's predicted
+    microseconds per PRN-step; `measure_top` > 0 then times that many of the
+    best candidates on the device and re-ranks them by the measured time.
+    """
+    system: Optional[GNSSSystem] = None
+    if isinstance(code_length_or_system, GNSSSystem):
+        system = code_length_or_system
+        code_length, code_freq_hz = system.get_code_length(), system.get_code_frequency()
+    else:
+        if code_freq is None:
+            raise ArgumentError("code_freq is required with an integer code_length")
+        code_length, code_freq_hz = int(code_length_or_system), float(code_freq)
+    sort_by = sort_by.lstrip(":")
+    if sort_by not in ("cost", "smoothness"):
+        raise ArgumentError(f"sort_by must be :cost or :smoothness, got :{sort_by}")
+    if not num_alternatives >= 1:
+        raise ArgumentError("num_alternatives must be >= 1")
+    if not max_prime >= 2:
+        raise ArgumentError("max_prime must be >= 2")
+    N = num_coherently_integrated_code_periods
+    if not N >= 1:
+        raise ArgumentError("num_coherently_integrated_code_periods must be >= 1")
+    fs_min_hz = code_freq_hz if fs_min is None else float(fs_min)
+    fs_max_hz = 1.5 * fs_min_hz if fs_max is None else float(fs_max)
+    step = float(fs_step)
+    if not fs_min_hz <= fs_max_hz:
+        raise ArgumentError("fs_min must be <= fs_max")
+    if not step > 0:
+        raise ArgumentError("fs_step must be > 0")
+    if sdr_clock_plan is not None:
+        lo, hi = sample_rate_range(sdr_clock_plan)
+        fs_min_hz, fs_max_hz = max(fs_min_hz, lo), min(fs_max_hz, hi)
+
+    T_code = code_length / code_freq_hz
+    seen = set()
+    cands: List[SamplingFreqRecommendation] = []
+    cov = float(min_doppler_coverage)
+    fs = fs_min_hz
+    while fs <= fs_max_hz:
+        if sdr_clock_plan is not None and not is_valid_sample_rate(sdr_clock_plan, fs):
+            fs += step
+            continue
+        spc = math.ceil(T_code * fs)
+        if spc not in seen:
+            seen.add(spc)
+            bin_width = fs / spc
+            nb = _smallest_valid_num_blocks(spc, math.ceil(2 * cov / bin_width + 2 / N))
+            if nb is not None:
+                bs = spc // nb
+                inner = nextprod((2, 3, 5, 7), 2 * bs)
+                ndop = N * nb
+                ndb_prime = _largest_prime_factor(ndop)
+                if ndb_prime <= max_prime:
+                    cands.append(SamplingFreqRecommendation(fs, spc, nb, bs, inner, ndop, nb * bin_width,
+                                                            _largest_prime_factor(inner), ndb_prime,
+                                                            _fft_cost(spc, nb, ndop, inner)))
+        fs += step
+
+    if cost_model is not None:
+        cands = [replace(c, cost=cost_model.cost(c)) for c in cands]
+    key = ((lambda c: (c.cost, c.num_doppler_bins_max_prime)) if sort_by == "cost"
+           else (lambda c: (c.num_doppler_bins_max_prime, c.cost)))
+    cands.sort(key=key)
+    if measure_top > 0 and cands:
+        cands = _measure_and_rerank(cands, measure_top, system, code_length, code_freq_hz, cov, N,
+                                    cost_model.nprn if cost_model is not None else 32, key)
+    return cands[:num_alternatives]
+
+
+def _measure_and_rerank(cands, k, system, code_length, code_freq_hz, cov, N, nprn, key):
+    """Time the k best candidates on the device; their cost becomes the
+    measured microseconds per PRN-step and they are re-sorted ahead of the rest."""
+    if system is None:
+        from .systems import SeededCodeSystem
+        system = SeededCodeSystem("seeded", code_length, code_freq_hz, 1575.42e6, 1, seed=0x5EED0006)
+    head = []
+    for c in cands[:k]:
+        m = GpuCostModel.measure(system, c.sampling_freq, N, nprn, min_doppler_coverage=cov)
+        if (m["spc"], m["nb"], m["F"], m["ndop"]) != (c.samples_per_code, c.num_blocks, c.inner_fft_size,
+                                                      c.num_doppler_bins):
+            raise RuntimeError(f"plan geometry at {c.sampling_freq} Hz differs from the recommender's: {m}")
+        head.append(replace(c, cost=1e6 * (m["corr_s"] + m["column_s"]) / nprn))
+    head.sort(key=key)
+    return head + cands[k:]

@@ -78,7 +78,6 @@ std::vector<float2> twiddles(int n)
     return tw;
 }
 
-// Radix schedule: 8s, then one 4 or 2, then 5s, 3s, 7s and larger primes.
 // Radix sequence of an n-point transform: 16s (when allowed), then 8, 4, 2,
 // 5, 3, then any larger prime.
 int make_fft_plan(int n, FftPlanDev* p, bool radix16)

@@ -1,0 +1,201 @@
+"""Sampling-frequency recommender and SDR clock plans (CPU only).
+
+The assertions follow test/recommend_sampling_freq.jl: geometry invariants
+of every candidate, sort orders, the column-transform smoothness gate
+(16.368 MHz recommended, 1.542 MHz rejected), argument validation, the
+AD9361 rule, and the factor-string known answers. The engine cost model's
+fit is checked against synthetic launch times of known coefficients.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from acquisition import GPSL1CA, ArgumentError
+from acquisition.plan import nextprod, plan_acquire
+from acquisition.recommend import (AD9361ClockPlan, GpuCostModel, SamplingFreqRecommendation, _factor_string,
+                                   _fft_cost, format_recommendations, is_valid_sample_rate,
+                                   recommend_sampling_freqs, sample_rate_range)
+
+
+def test_code_length_form_geometry():
+    """fs 36-40 MHz, 10000-chip code at 36 MHz, ±10 kHz, N_coh 36
+    (test/recommend_sampling_freq.jl:4-44)."""
+    rs = recommend_sampling_freqs(10_000, 36e6, fs_min=36e6, fs_max=40e6, min_doppler_coverage=10_000.0,
+                                  num_coherently_integrated_code_periods=36, num_alternatives=5, max_prime=7,
+                                  sort_by="cost", fs_step=1000.0)
+    assert 1 <= len(rs) <= 5
+    assert all(isinstance(r, SamplingFreqRecommendation) for r in rs)
+    for r in rs:
+        assert r.inner_max_prime <= 7 and r.num_doppler_bins_max_prime <= 7
+        assert r.samples_per_code % r.num_blocks == 0
+        assert r.block_size == r.samples_per_code // r.num_blocks
+        assert r.inner_fft_size == nextprod((2, 3, 5, 7), 2 * r.block_size) >= 2 * r.block_size
+        assert r.num_doppler_bins == 36 * r.num_blocks
+        assert r.doppler_coverage >= 2 * 10_000
+        assert r.cost == _fft_cost(r.samples_per_code, r.num_blocks, r.num_doppler_bins, r.inner_fft_size)
+    costs = [r.cost for r in rs]
+    assert costs == sorted(costs)
+
+
+def test_smoothness_sort():
+    rs = recommend_sampling_freqs(10_000, 36e6, fs_min=36e6, fs_max=40e6, min_doppler_coverage=10_000.0,
+                                  num_coherently_integrated_code_periods=36, num_alternatives=5,
+                                  sort_by=":smoothness")
+    assert rs
+    primes = [r.num_doppler_bins_max_prime for r in rs]
+    assert primes == sorted(primes)
+
+
+def test_column_transform_is_the_only_gate():
+    """16368 = 2⁴·3·11·31 is not smooth, but nb 16 gives a smooth column
+    transform and the inner transform pads 2046 -> 2048; 1542 = 2·3·257 only
+    offers nb 257 (test/recommend_sampling_freq.jl:63-89)."""
+    rs = recommend_sampling_freqs(GPSL1CA(), fs_min=16.36e6, fs_max=16.40e6, min_doppler_coverage=7000.0,
+                                  num_alternatives=20)
+    assert 16_368_000 in [round(r.sampling_freq) for r in rs]
+    for r in rs:
+        assert r.num_doppler_bins_max_prime <= 7 and r.inner_max_prime <= 7
+        assert r.inner_fft_size >= 2 * r.block_size
+    r = next(r for r in rs if round(r.sampling_freq) == 16_368_000)
+    assert (r.samples_per_code, r.num_blocks, r.block_size, r.inner_fft_size) == (16368, 16, 1023, 2048)
+    rs2 = recommend_sampling_freqs(GPSL1CA(), fs_min=1.540e6, fs_max=1.545e6, min_doppler_coverage=7000.0,
+                                   num_alternatives=20)
+    assert all(round(r.sampling_freq) != 1_542_000 for r in rs2)
+
+
+def test_system_form_matches_plan_acquire():
+    """samples_per_code = ceil(fs/1000) for L1 C/A, and every candidate's
+    geometry equals what plan_acquire derives at that rate."""
+    rs = recommend_sampling_freqs(GPSL1CA(), fs_min=2e6, fs_max=5e6, min_doppler_coverage=7000.0,
+                                  num_coherently_integrated_code_periods=1, num_alternatives=3)
+    assert 1 <= len(rs) <= 3
+    assert rs[0].samples_per_code == math.ceil(rs[0].sampling_freq / 1000)
+    from acquisition.plan import _plan_host
+    for r in rs:
+        p = _plan_host(GPSL1CA(), r.sampling_freq, [1])
+        assert (p.samples_per_code, p.num_blocks, p.block_size, p.fft_size, p.num_doppler_bins) == (
+            r.samples_per_code, r.num_blocks, r.block_size, r.inner_fft_size, r.num_doppler_bins)
+        assert p.doppler_freqs[0] == pytest.approx(-r.doppler_coverage / 2, rel=1e-12)
+
+
+def test_num_alternatives_clamps():
+    assert len(recommend_sampling_freqs(GPSL1CA(), fs_min=2e6, fs_max=10e6, num_alternatives=2)) <= 2
+
+
+@pytest.mark.parametrize("kw", [dict(sort_by="nonsense"), dict(num_alternatives=0), dict(max_prime=1),
+                                dict(num_coherently_integrated_code_periods=0), dict(fs_min=5e6, fs_max=1e6),
+                                dict(fs_step=0.0)])
+def test_argument_validation(kw):
+    with pytest.raises(ArgumentError):
+        recommend_sampling_freqs(1023, 1.023e6, **kw)
+
+
+def test_ad9361_rule():
+    plan = AD9361ClockPlan()
+    for fs in (5e6, 10e6, 20e6, 30.72e6, 61.44e6, 25e6, 2.5e6):
+        assert is_valid_sample_rate(plan, fs)
+    for fs in (100e3, 70e6, 1e6):
+        assert not is_valid_sample_rate(plan, fs)
+    assert sample_rate_range(plan) == (0.55e6, 61.44e6)
+    assert "AD9361ClockPlan" in str(plan) and "dividers" in str(plan)
+
+
+def test_sdr_clock_plan_filters_and_clamps():
+    plan = AD9361ClockPlan()
+    free = recommend_sampling_freqs(GPSL1CA(), fs_min=2e6, fs_max=5e6, num_alternatives=5)
+    held = recommend_sampling_freqs(GPSL1CA(), fs_min=2e6, fs_max=5e6, num_alternatives=5, sdr_clock_plan=plan)
+    assert all(is_valid_sample_rate(plan, r.sampling_freq) for r in held)
+    assert len(held) <= len(free)
+    top = recommend_sampling_freqs(GPSL1CA(), fs_min=60e6, fs_max=80e6, num_alternatives=5, sdr_clock_plan=plan)
+    assert all(r.sampling_freq <= 61.44e6 for r in top)
+
+
+def test_factor_string_known_answers():
+    for n, s in [(1, "1"), (2, "2"), (8, "2³"), (744, "2³·3·31"), (2048, "2¹¹"), (10080, "2⁵·3²·5·7")]:
+        assert _factor_string(n) == s
+
+
+def test_show_forms():
+    rs = recommend_sampling_freqs(GPSL1CA(), fs_min=2e6, fs_max=5e6, num_alternatives=2)
+    table = format_recommendations(rs)
+    for label in ("Sampling freq (MHz)", "Inner FFT size", "Num Doppler bins", "Estimated cost"):
+        assert label in table
+    assert "omitted" not in table and "(" in table
+    one = str(rs[0])
+    assert "SamplingFreqRecommendation" in one and "samples_per_code" in one and "inner_fft=" in one
+    assert "no candidates" in format_recommendations([])
+
+
+# -- engine cost model ----------------------------------------------------------------
+TRUE = GpuCostModel(corr={"f2048": (4e-6, 3.0e-14), "f2500": (4e-6, 6.0e-14), "generic": (6e-6, 8.0e-14)},
+                    column={"c160": (5e-6, 2.0e-12, 0.0), "generic": (5e-6, 1.5e-12, 6.0e-14)})
+
+
+def test_engine_radix_plans():
+    """The radix sequence make_fft_plan builds: 16s when allowed, 8s, 4, 2,
+    5s, 3s, larger primes; a direct prime stage costs 14·R per point."""
+    from acquisition.recommend import _engine_radices, _transform_work
+    assert _engine_radices(2048, True) == [16, 16, 8]
+    assert _engine_radices(2048, False) == [8, 8, 8, 4]
+    assert _engine_radices(504, True) == [8, 3, 3, 7]
+    assert _engine_radices(2744, True) == [8, 7, 7, 7]
+    assert _engine_radices(160, False) == [8, 4, 5]
+    assert _engine_radices(125, False) == [5, 5, 5]
+    assert _engine_radices(1, False) == []
+    assert _transform_work(504, True) == 504 * (12.25 + 8.7 + 8.7 + 98.0)
+    for n in (16, 20, 500, 504, 1000, 1250, 1600, 2048, 2500, 2744):
+        assert np.prod(_engine_radices(n, True)) == n
+
+
+def _synthetic_points():
+    pts = []
+    for fs, N in GpuCostModel.DEFAULT_GEOMETRIES:
+        p = _plan_geom(fs, N)
+        for n in (8, 32):
+            tc, tk = TRUE.predict_launches(p["spc"], p["nb"], p["F"], p["ndop"], n)
+            pts.append(dict(p, nprn=n, corr_s=tc, column_s=tk))
+    return pts
+
+
+def _plan_geom(fs, N):
+    from acquisition.plan import _plan_host
+    p = _plan_host(GPSL1CA(), fs, [1], num_coherently_integrated_code_periods=N)
+    return dict(fs=fs, N=N, spc=p.samples_per_code, nb=p.num_blocks, F=p.fft_size, ndop=p.num_doppler_bins)
+
+
+def test_default_geometries_cover_every_variant():
+    pts = _synthetic_points()
+    from acquisition.recommend import _column_variant, _corr_variant
+    assert {_corr_variant(p["F"]) for p in pts} == {"f2048", "f2500", "generic"}
+    assert {_column_variant(p["ndop"], p["spc"] // p["nb"]) for p in pts} == {"c160", "generic"}
+
+
+def test_fit_recovers_known_coefficients():
+    fit = GpuCostModel.fit(_synthetic_points())
+    for k, v in TRUE.corr.items():
+        np.testing.assert_allclose(fit.corr[k], v, rtol=1e-6, atol=1e-15)
+    for k, v in TRUE.column.items():
+        np.testing.assert_allclose(fit.column[k], v, rtol=1e-6, atol=1e-15)
+
+
+def test_model_cost_ranks_candidates():
+    """Under a cost model the cost is microseconds per PRN-step, sorted."""
+    rs = recommend_sampling_freqs(GPSL1CA(), fs_min=4e6, fs_max=17e6, num_alternatives=8, cost_model=TRUE)
+    assert rs and [r.cost for r in rs] == sorted(r.cost for r in rs)
+    for r in rs:
+        want = 1e6 * TRUE.predict_step(r.samples_per_code, r.num_blocks, r.inner_fft_size, r.num_doppler_bins)
+        assert r.cost == pytest.approx(want, rel=1e-12)
+
+
+def test_recorded_mi355x_table_matches_l1ca_profile():
+    """The recorded MI355X coefficients predict the L1 C/A 16.368 MHz, 32 PRN,
+    N_coh 10 launches of the committed kernel profile (k_corr_tile 0.2548 ms,
+    k_column_c160 0.2642 ms per launch, mean over 8 noncoherent steps) within
+    15 %, and a sweep ranked under them comes back sorted by predicted cost."""
+    m = GpuCostModel.mi355x()
+    tc, tk = m.predict_launches(16368, 16, 2048, 160, 32)
+    assert abs(tc / 0.2548e-3 - 1) < 0.15 and abs(tk / 0.2642e-3 - 1) < 0.15
+    rs = recommend_sampling_freqs(GPSL1CA(), fs_min=16.0e6, fs_max=16.5e6, num_coherently_integrated_code_periods=10,
+                                  num_alternatives=50, cost_model=m)
+    assert rs == sorted(rs, key=lambda r: (r.cost, r.num_doppler_bins_max_prime))
