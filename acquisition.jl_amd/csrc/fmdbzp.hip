@@ -195,6 +195,8 @@ struct fmdbzp_plan {
     size_t fft_lds = 0;
     size_t corr_lds = 0;
     float inv_f = 0.f;
+    float pscale = 1.f;          // factor in the conj code spectra (1/F when F is a power of two)
+    float tile_scale = 0.f;      // factor the block correlation applies (1/F otherwise)
     // timing
     bool timing = false;
     std::vector<TimedEvent> pending;
@@ -468,16 +470,27 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     const int F = d.fft_size, ndop = d.num_doppler_bins, nb = d.num_blocks, bs = d.block_size;
     const int spc = d.samples_per_code, spc_eff = d.samples_per_code_eff;
     P->inv_f = 1.0f / (float)F;
+    // Power-of-two F: 1/F is exact and goes into the conj code spectra, so the
+    // correlation's inverse FFT needs no scaling (same values: multiplying by a
+    // power of two commutes exactly with every add and multiply of the FFT).
+    const bool f_pow2 = (F & (F - 1)) == 0;
+    P->pscale = f_pow2 ? P->inv_f : 1.0f;
+    P->tile_scale = f_pow2 ? 1.0f : P->inv_f;
     // Row pitches rounded to 32 elements: every tile row and accumulator row
     // starts on a 256 / 128-byte boundary, so a workgroup's column segment
     // does not straddle more cache lines than its size needs.
     P->tp = (bs + 31) & ~31;
     P->ap = (spc_eff + 31) & ~31;
+    // the correlation addresses one PRN's code spectra and one pass's tile
+    // rows by 32-bit byte offsets
+    if ((size_t)nb * F * sizeof(float2) >= (1ull << 31) || (size_t)2 * ndop * P->tp * sizeof(float2) >= (1ull << 31))
+        return fail(set_err(FMDBZP_EINVAL, "geometry too large: nb*F = %lld, ndop*bs = %lld", (long long)nb * F,
+                            (long long)ndop * bs));
     P->fft_lds = (size_t)2 * fft_lds_span(F) * sizeof(float2);
     // block correlation: the staged signal-FFT row plus, with in-place
     // stages, one buffer per concurrent transform (two when they fit the
     // register bound), else two ping-pong buffers
-    if (F == 2048) P->corr_variant = 3;  // compile-time 16 x 16 x 8 transforms, two per pass
+    if (F == 2048) P->corr_variant = 3;  // compile-time 16 x 16 x 8 transforms, two per pass, unscaled
     else if (F == 2500) P->corr_variant = 4;  // compile-time 10 x 10 x 5 x 5, one per pass
     else if (fft_inplace_ok(P->pf) && 2 * F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 2;
     else if (fft_inplace_ok(P->pf) && F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 1;
@@ -619,7 +632,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         if (e == hipSuccess) {
             Timed t(P, FMDBZP_K_CODE_FFT, P->stream);
             k_code_fft<<<dim3(nb, nprn), FMDBZP_NTHR, P->fft_lds, P->stream>>>(d_codes, P->d_prnconj, P->pf8,
-                                                                              P->d_twF, spc, nb, bs);
+                                                                              P->d_twF, spc, nb, bs, P->pscale);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
@@ -815,19 +828,19 @@ void launch_corr(fmdbzp_plan* P, dim3 g, hipStream_t st, const float2* sf, const
 {
     if (P->corr_variant == 4)
         k_corr_tile<1, true, 2500, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);
     else if (P->corr_variant == 3)
         k_corr_tile<2, true, 2048, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);
     else if (P->corr_variant == 2)
         k_corr_tile<2, true, 0, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);
     else if (P->corr_variant == 1)
         k_corr_tile<1, true, 0, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);
     else
         k_corr_tile<1, false, 0, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
-            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->inv_f, hscale);
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);
 }
 
 // Enqueue one search on the plan's stream; statistics land in the plan's

@@ -92,14 +92,20 @@ struct CodeIn {
         return make_float2(i < bs ? (float)code[i] : 0.f, 0.f);
     }
 };
+// conj(v) times scale: 1, or 1/F when F is a power of two (then exact, and the
+// block correlation's own 1/F factor is applied here once per plan).
 struct ConjOut {
     float2* out;
-    __device__ __forceinline__ void operator()(int, int i, float2 v) const { out[i] = make_float2(v.x, -v.y); }
+    float scale;
+    __device__ __forceinline__ void operator()(int, int i, float2 v) const
+    {
+        out[i] = make_float2(v.x * scale, -v.y * scale);
+    }
 };
 
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_code_fft(const int8_t* __restrict__ codes, float2* __restrict__ prnconj, FftPlanDev pf,
-           const float2* __restrict__ twF, int spc, int nb, int bs)
+           const float2* __restrict__ twF, int spc, int nb, int bs, float pscale)
 {
     extern __shared__ float2 lds[];
     const int F = pf.n;
@@ -108,7 +114,7 @@ k_code_fft(const int8_t* __restrict__ codes, float2* __restrict__ prnconj, FftPl
     float2* out = prnconj + ((size_t)prn * nb + blk) * F;
     const int span = fft_lds_span(F);
     fft_run<false, false, CodeIn, ConjOut, false>(lds, lds + span, pf, twF, 1, span, threadIdx.x, FMDBZP_NTHR,
-                                                  CodeIn{code, bs}, ConjOut{out});
+                                                  CodeIn{code, bs}, ConjOut{out, pscale});
 }
 
 // ---------------------------------------------------------------------------
@@ -181,15 +187,20 @@ struct CorrIn {
     {
         int b = b0 + t;
         if (b >= nb) b -= nb;
-        return cmulf(S[i], P[(size_t)b * F + i]);
+        // 32-bit byte offset from the PRN's base (nb * F * 8 < 2^31, checked at
+        // plan creation): the address is one vector add on a scalar base
+        const uint32_t off = (uint32_t)(b * F + i) * (uint32_t)sizeof(float2);
+        return cmulf(S[i], *(const float2*)((const char*)P + off));
     }
 };
 // TileT float2: FP32 tile. TileT __half2: FP16 tile of the values times
-// hscale, a power of two (FMDBZP_PREC_TILE_FP16).
-template <class TileT>
+// hscale, a power of two (FMDBZP_PREC_TILE_FP16). SCALE: multiply by inv_f
+// here; without it the 1/F factor is already in the code spectra (power-of-two
+// F), which gives the same values exactly.
+template <class TileT, bool SCALE = true>
 struct LagOut {
     TileT* out;         // tile row of transform 0
-    size_t tstride;     // tile distance between consecutive column blocks
+    uint32_t tstride;   // tile distance between consecutive column blocks (elements)
     int bs;
     float inv_f;
     int nt;             // transforms of this pass that exist (the rest are padding)
@@ -197,16 +208,20 @@ struct LagOut {
     __device__ __forceinline__ void operator()(int t, int i, float2 v) const
     {
         if (i < bs && t < nt) {
+            const float sx = SCALE ? v.x * inv_f : v.x, sy = SCALE ? v.y * inv_f : v.y;
+            // 32-bit byte offset from the pass's base (the tile of one PRN is
+            // below 2^31 bytes, checked at plan creation)
+            char* dst = (char*)out + ((uint32_t)t * tstride + (uint32_t)i) * (uint32_t)sizeof(TileT);
             // the tile is written once and read by the next kernel: nontemporal stores
             if constexpr (sizeof(TileT) == sizeof(float2)) {
                 typedef float f2v __attribute__((ext_vector_type(2)));
                 f2v u;
-                u.x = v.x * inv_f;
-                u.y = v.y * inv_f;
-                __builtin_nontemporal_store(u, (f2v*)&out[t * tstride + i]);
+                u.x = sx;
+                u.y = sy;
+                __builtin_nontemporal_store(u, (f2v*)dst);
             } else {
-                const __half2 h = __floats2half2_rn((v.x * inv_f) * hscale, (v.y * inv_f) * hscale);
-                __builtin_nontemporal_store(*(const uint32_t*)&h, (uint32_t*)&out[t * tstride + i]);
+                const __half2 h = __floats2half2_rn(sx * hscale, sy * hscale);
+                __builtin_nontemporal_store(*(const uint32_t*)&h, (uint32_t*)dst);
             }
         }
     }
@@ -217,8 +232,9 @@ struct LagOut {
 // stages, one LDS buffer per concurrent transform; else two ping-pong buffers
 // (NT = 1).
 // grid (ndop, batch)
-// STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass; STATIC =
-// 2500: 10 x 10 x 5 x 5 (the E1-like search at 25 MHz).
+// STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass, with the
+// 1/F factor taken from the code spectra; STATIC = 2500: 10 x 10 x 5 x 5 (the
+// E1-like search at 25 MHz). inv_f is 1 when the code spectra carry 1/F.
 template <int NT, bool ONEBUF, int STATIC = 0, class TileT = float2>
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
@@ -245,7 +261,8 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
         int b0 = kb + r0;
         if (b0 >= nb) b0 -= nb;
         CorrIn in{srow, Pp, F, nb, b0};
-        LagOut<TileT> out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (size_t)ndop * tp, bs, inv_f, nt, hscale};
+        LagOut<TileT, STATIC != 2048> out{tile + (((size_t)p * nb + r0) * ndop + k) * tp, (uint32_t)(ndop * tp), bs,
+                                          inv_f, nt, hscale};
         if constexpr (STATIC == 2048)
             fft3_static<true, 2048, NT, FMDBZP_NTHR, 16, 16, 8>(a, twF, tid, in, out);
         else if constexpr (STATIC == 2500)
