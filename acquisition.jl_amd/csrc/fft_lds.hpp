@@ -41,7 +41,9 @@ __device__ __forceinline__ float2 mul_mi(float2 a)
 template <bool INV>
 __device__ __forceinline__ float2 twid(const float2* __restrict__ tw, int idx)
 {
-    float2 w = tw[idx];
+    // 32-bit byte offset from the table's base (a table holds one transform's
+    // n entries): one vector add on a scalar base instead of a 64-bit address
+    float2 w = *(const float2*)((const char*)tw + (uint32_t)idx * (uint32_t)sizeof(float2));
     if (INV) w.y = -w.y;
     return w;
 }

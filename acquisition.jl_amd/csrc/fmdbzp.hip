@@ -548,7 +548,9 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // every scatter into isolated 4-byte updates; those plans keep the
     // accumulator column-major, with rows contiguous per code column.
     P->acc_cm = !P->simple_kernel && LB < 16;
-    P->rot2000 = P->rot_direct && ndop == 2000 && d.num_coh <= 32;
+    // k_column_rot2000 addresses a rotation slice of the accumulator by 32-bit byte offsets
+    P->rot2000 = P->rot_direct && ndop == 2000 && d.num_coh <= 32 &&
+                 (size_t)spc * ((ndop + 31) & ~31) * sizeof(float) < (1ull << 32);
     if (P->rot2000) {
         P->acc_cm = true;
         LB = P->LB = 1;

@@ -787,7 +787,8 @@ k_column_rot2000(ColumnArgs A)
             if (row >= N) row -= N;
             int dc = pc + A.shifts[row];
             if (dc >= A.spc) dc -= A.spc;
-            off[it][i] = (uint32_t)(row + dc * A.acc_ldc);
+            // byte offset within a rotation slice (below 2^32, checked at plan creation)
+            off[it][i] = (uint32_t)(row + dc * A.acc_ldc) * (uint32_t)sizeof(float);
         }
     }
     const uint32_t magic = (uint32_t)((0x100000000ull + (uint64_t)A.nb - 1) / (uint64_t)A.nb);
@@ -799,16 +800,27 @@ k_column_rot2000(ColumnArgs A)
         const int nq = min(2, A.R - q0);
         float* base0 = accp + (size_t)q0 * slice_stride;
         float* base1 = base0 + slice_stride;
+        // Opaque per-pass copies of the byte offsets: the loads and stores then
+        // see scalar base + 32-bit offset and take the scalar-base address
+        // form, instead of 64-bit adds on offsets widened once outside the loop.
+        uint32_t po[IT3][5];
+#pragma unroll
+        for (int it = 0; it < IT3; it++)
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                po[it][i] = off[it][i];
+                asm volatile("" : "+v"(po[it][i]));
+            }
         float prev[IT3][2][5];
         if (!first) {
 #pragma unroll
             for (int it = 0; it < IT3; it++) {
                 if (tid + it * FMDBZP_NTHR < M3) {
 #pragma unroll
-                    for (int i = 0; i < 5; i++) prev[it][0][i] = base0[off[it][i]];
+                    for (int i = 0; i < 5; i++) prev[it][0][i] = *(const float*)((const char*)base0 + po[it][i]);
                     if (nq > 1) {
 #pragma unroll
-                        for (int i = 0; i < 5; i++) prev[it][1][i] = base1[off[it][i]];
+                        for (int i = 0; i < 5; i++) prev[it][1][i] = *(const float*)((const char*)base1 + po[it][i]);
                     }
                 }
             }
@@ -845,12 +857,12 @@ k_column_rot2000(ColumnArgs A)
                 for (int u = 0; u < 2; u++) {
                     if (u < nq) {
                         bfly<5, false>(v[u]);
-                        float* b = u ? base1 : base0;
+                        char* b = (char*)(u ? base1 : base0);
 #pragma unroll
                         for (int i = 0; i < 5; i++) {
                             // muladd(re, re, im * im), the reference's rotation power
                             const float pw = fmaf(v[u][i].x, v[u][i].x, __fmul_rn(v[u][i].y, v[u][i].y));
-                            b[off[it][i]] = first ? pw : prev[it][u][i] + pw;
+                            *(float*)(b + po[it][i]) = first ? pw : prev[it][u][i] + pw;
                         }
                     }
                 }
@@ -890,16 +902,30 @@ k_reduce_rows(const float* __restrict__ acc, int ndop, int spc, int spc_eff, int
     __shared__ float sv[FMDBZP_NTHR];
     __shared__ uint64_t sk[FMDBZP_NTHR];
     const int row = blockIdx.x, p = blockIdx.y, tid = threadIdx.x;
-    const float* a = acc + ((size_t)p * ndop + row) * ap;
+    const float* a = acc + ((size_t)p * ndop + row) * ap;  // 128-byte aligned (ap is a multiple of 32)
     double s = 0.0;
     float bv = -INFINITY;
     uint64_t bk = ~0ull;
-    for (int c = tid; c < spc_eff; c += FMDBZP_NTHR) {
-        const float v = a[c];
+    // the key is only needed when the value can win (v >= bv)
+    auto visit = [&](float v, int c) {
         s += (double)v;
-        const uint64_t k = cell_key(row, c, ndop, spc, R, streamed_rot);
-        if (better(v, k, bv, bk)) { bv = v; bk = k; }
+        if (v >= bv) {
+            const uint64_t k = cell_key(row, c, ndop, spc, R, streamed_rot);
+            if (better(v, k, bv, bk)) { bv = v; bk = k; }
+        }
+    };
+    // 16-byte loads, four in flight per thread
+    const int n4 = spc_eff >> 2;
+    const float4* a4 = (const float4*)a;
+#pragma unroll 4
+    for (int g = tid; g < n4; g += FMDBZP_NTHR) {
+        const float4 v = a4[g];
+        visit(v.x, 4 * g);
+        visit(v.y, 4 * g + 1);
+        visit(v.z, 4 * g + 2);
+        visit(v.w, 4 * g + 3);
     }
+    for (int c = 4 * n4 + tid; c < spc_eff; c += FMDBZP_NTHR) visit(a[c], c);
     ssum[tid] = s;
     sv[tid] = bv;
     sk[tid] = bk;
