@@ -537,7 +537,9 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     if (LB > bs) LB = bs;
     // ndop 160 (and a column block of at least 32 lags): compile-time 10 x 16
     // column kernel with a fixed column count (k_column_c160)
-    P->col_static = P->simple_kernel && ndop == 160 && bs >= 32;
+    // k_column_c160 addresses one PRN's accumulator and one tile block by 32-bit byte offsets
+    P->col_static = P->simple_kernel && ndop == 160 && bs >= 32 &&
+                    (size_t)ndop * P->ap * sizeof(float) < (1ull << 31);
     if (P->col_static) LB = 32;
     P->tile_half = d.precision == FMDBZP_PREC_TILE_FP16;
     if (P->tile_half && !P->col_static)

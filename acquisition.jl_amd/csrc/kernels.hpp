@@ -386,16 +386,21 @@ k_column_c160(ColumnArgs A, float hinv)
     const int t1 = tid % LBS, j1 = tid / LBS;
     const bool act1 = tid < LBS * M1 && t1 < ncol;
     const int cp = r * A.bs + lag0 + t1;
+    // byte offsets of this thread's 16 accumulator cells from the PRN's base
+    // (one PRN's accumulator is below 2^31 bytes, checked at plan creation)
+    uint32_t aoff[R1];
+#pragma unroll
+    for (int q = 0; q < R1; q++) {
+        int row = j1 + M1 * q + N / 2;
+        if (row >= N) row -= N;
+        int dc = cp + A.shifts[row];
+        if (dc >= A.spc) dc -= A.spc;
+        aoff[q] = (uint32_t)(row * A.ap + dc) * (uint32_t)sizeof(float);
+    }
     float accv[R1];
     if (act1 && !A.first_step) {
 #pragma unroll
-        for (int q = 0; q < R1; q++) {
-            int row = j1 + M1 * q + N / 2;
-            if (row >= N) row -= N;
-            int dc = cp + A.shifts[row];
-            if (dc >= A.spc) dc -= A.spc;
-            accv[q] = accp[(size_t)row * A.ap + dc];
-        }
+        for (int q = 0; q < R1; q++) accv[q] = *(const float*)((const char*)accp + aoff[q]);
     }
 
     // stage 1: radix 10 over rows j + 16 m, straight from the tile
@@ -416,9 +421,12 @@ k_column_c160(ColumnArgs A, float hinv)
                         v[m] = make_float2(u.x * hinv, u.y * hinv);
                     }
                 } else {
-                    // the tile is read once: nontemporal, so it does not displace accumulator lines
+                    // the tile is read once: nontemporal, so it does not displace accumulator lines;
+                    // 32-bit byte offsets from the block's base (a tile block is below 2^31 bytes)
 #pragma unroll
-                    for (int m = 0; m < R0; m++) v[m] = load_tile_nt(&T[(size_t)(j + M0 * m) * A.tp + t]);
+                    for (int m = 0; m < R0; m++)
+                        v[m] = load_tile_nt((const float2*)((const char*)T +
+                                                            (uint32_t)((j + M0 * m) * A.tp + t) * (uint32_t)sizeof(float2)));
                 }
                 bfly10<false>(v);
 #pragma unroll
@@ -433,16 +441,12 @@ k_column_c160(ColumnArgs A, float hinv)
 #pragma unroll
         for (int q = 0; q < R1; q++) v[q] = lds[t1 * bst + j1 + M1 * q];
 #pragma unroll
-        for (int q = 1; q < R1; q++) v[q] = cmulf(v[q], A.twD[q * j1]);
+        for (int q = 1; q < R1; q++) v[q] = cmulf(v[q], twid<false>(A.twD, q * j1));
         bfly16<false>(v);
 #pragma unroll
         for (int q = 0; q < R1; q++) {
-            int row = j1 + M1 * q + N / 2;
-            if (row >= N) row -= N;
-            int dc = cp + A.shifts[row];
-            if (dc >= A.spc) dc -= A.spc;
             const float pw = abs2_exact(v[q]);
-            accp[(size_t)row * A.ap + dc] = A.first_step ? pw : accv[q] + pw;
+            *(float*)((char*)accp + aoff[q]) = A.first_step ? pw : accv[q] + pw;
         }
     }
 }
