@@ -9,6 +9,7 @@ the secondary-code phase gate and estimator).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from dataclasses import dataclass
 from typing import Any, Optional, Sequence, Union
@@ -208,61 +209,25 @@ class _SignalView:
 # ---------------------------------------------------------------------------
 # acquire_
 # ---------------------------------------------------------------------------
-def _extract(plan: AcquisitionPlan, st, prn: int, sigview: _SignalView, if_hz: float,
-             subsample_interpolation: bool, power_bins) -> AcquisitionResults:
-    """Result extraction (src/acquire.jl:544-671 and :681-781) from the engine's
-    per-PRN statistics."""
-    system = plan.system
+def _interpolate(plan: AcquisitionPlan, peak, noise, code_nbrs, dop_nbrs, delay: int, code_phase: float,
+                 doppler: float):
+    """Subsample interpolation of one peak (src/acquire.jl:593-619, :714-748): a parabola
+    through the code (Doppler) neighbours when the larger one exceeds sqrt(noise)."""
     fs = plan.sampling_freq
-    fc = system.get_code_frequency()
-    code_length = system.get_code_length()
-    ndop = plan.num_doppler_bins
-    spc = plan.samples_per_code
-    peak = f32(st.peak_power)
-    noise = f32(st.noise_power)
-    signal_power = f32(peak - noise)
-    ptn = f32(f32(signal_power + noise) / noise)
-    code_period = code_length / fc
-    tcoh = plan.num_coherently_integrated_code_periods * code_period
-    with np.errstate(divide="ignore", invalid="ignore"):
-        ratio = float(f32(signal_power / noise))
-    cn0 = 10 * math.log10(ratio / tcoh / 1.0) if ratio > 0 else (-math.inf if ratio == 0 else math.nan)
-
-    row, col = int(st.peak_row), int(st.peak_col)
-    rotation_block = col // spc
-    scrambled = col % spc
-    delay = fmdbzp_column_to_tau(scrambled, plan.num_blocks, plan.block_size)
-    code_phase = jl_mod(-delay * fc / fs, code_length)
-    doppler = float(plan.doppler_freqs[row])
-    if subsample_interpolation:
-        pl, pr = f32(st.code_left), f32(st.code_right)
-        dl, dr = f32(st.dop_left), f32(st.dop_right)
-        sq = np.sqrt(noise)
-        if max(pl, pr) > sq:
-            frac = parabolic_interp(pl, peak, pr)
-            if isinstance(frac, np.float32):
-                di = f32(f32(delay) + frac)
-                code_phase = jl_mod(float(-di) * fc / fs, code_length)
-        if max(dl, dr) > sq:
-            frac = parabolic_interp(dl, peak, dr)
-            doppler = doppler + float(frac) * plan.doppler_spacing_hz
-
-    # the secondary-code phase of a detected PRN (src/acquire.jl:646-652) is
-    # filled in by _results_from_stats, for all such PRNs in one device call
-    secondary_code_phase = None
-    needs_secondary = False
-    if plan.num_secondary_rotations > 1:
-        num_cells = ndop * plan.samples_per_code * plan.num_secondary_rotations
-        thr = cfar_threshold(0.01, num_cells, num_noncoherent_integrations=plan.num_noncoherent_accumulations)
-        needs_secondary = float(ptn) > thr
-
-    r = AcquisitionResults(
-        system=system, prn=prn, sampling_frequency=fs, carrier_doppler=doppler, code_phase=code_phase,
-        secondary_code_phase=secondary_code_phase, CN0=cn0, noise_power=noise, peak_to_noise_ratio=ptn,
-        num_noncoherent_integrations=plan.num_noncoherent_accumulations, power_bins=power_bins,
-        dopplers=plan.doppler_freqs, num_blocks=plan.num_blocks, block_size=plan.block_size,
-        num_secondary_rotations=plan.num_secondary_rotations)
-    return r, needs_secondary
+    fc = plan.system.get_code_frequency()
+    code_length = plan.system.get_code_length()
+    pl, pr = f32(code_nbrs[0]), f32(code_nbrs[1])
+    dl, dr = f32(dop_nbrs[0]), f32(dop_nbrs[1])
+    sq = np.sqrt(noise)
+    if max(pl, pr) > sq:
+        frac = parabolic_interp(pl, peak, pr)
+        if isinstance(frac, np.float32):
+            di = f32(f32(delay) + frac)
+            code_phase = jl_mod(float(-di) * fc / fs, code_length)
+    if max(dl, dr) > sq:
+        frac = parabolic_interp(dl, peak, dr)
+        doppler = doppler + float(frac) * plan.doppler_spacing_hz
+    return code_phase, doppler
 
 
 def _get_result_buffer(plan: AcquisitionPlan, prn_idx: int) -> np.ndarray:
@@ -326,25 +291,73 @@ def _prepare(plan: AcquisitionPlan, signal, prns, interm_freq, store_power_bins)
     return prns, sv, idx, flags, bufs, surfaces
 
 
+# numpy view of fmdbzp_prn_stats (the ctypes PrnStats layout), built once
+_STATS_DTYPE = np.dtype({"names": [f[0] for f in _native.PrnStats._fields_],
+                         "formats": [np.dtype(f[1]) for f in _native.PrnStats._fields_],
+                         "offsets": [getattr(_native.PrnStats, f[0]).offset for f in _native.PrnStats._fields_],
+                         "itemsize": ctypes.sizeof(_native.PrnStats)})
+
+
 def _results_from_stats(plan, stats, prns, sv, interm_freq, subsample_interpolation, bufs):
+    """Result extraction (src/acquire.jl:544-671 and :681-781) from the
+    engine's per-PRN statistics, all PRNs of the call at once. The Float32
+    steps (signal power, peak-to-noise ratio, the CN0 ratio) are numpy float32
+    array operations, each rounded to Float32 as the reference's scalar code
+    does; the code phase and Doppler follow in Float64 per PRN."""
+    n = len(prns)
+    a = np.frombuffer(stats, dtype=_STATS_DTYPE, count=n)
+    system = plan.system
+    fs = plan.sampling_freq
+    fc = system.get_code_frequency()
+    code_length = system.get_code_length()
+    spc, nb, bsz = plan.samples_per_code, plan.num_blocks, plan.block_size
+    peak = np.ascontiguousarray(a["peak_power"], dtype=np.float32)
+    noise = np.ascontiguousarray(a["noise_power"], dtype=np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        signal_power = peak - noise
+        ptn = (signal_power + noise) / noise
+        ratio = (signal_power / noise).tolist()
+    code_period = code_length / fc
+    tcoh = plan.num_coherently_integrated_code_periods * code_period
+    rows, cols = a["peak_row"].tolist(), a["peak_col"].tolist()
+    peaks, noises = peak.tolist(), noise.tolist()
+    nbrs = np.stack([a["code_left"], a["code_right"], a["dop_left"], a["dop_right"]], axis=1).tolist()
+    nsums, ncells = a["noise_sum"].tolist(), a["noise_cells"].tolist()
+    # the secondary-code phase of a detected PRN (src/acquire.jl:646-652) is
+    # estimated for all such PRNs in one device call below
+    R = plan.num_secondary_rotations
+    thr = None
+    if R > 1:
+        thr = cfar_threshold(0.01, plan.num_doppler_bins * spc * R,
+                             num_noncoherent_integrations=plan.num_noncoherent_accumulations)
     results, pending = [], []
     for j, p in enumerate(prns):
-        r, needs_secondary = _extract(plan, stats[j], p, sv, float(interm_freq), subsample_interpolation,
-                                      bufs[j] if bufs is not None else None)
-        if needs_secondary:
+        rj = ratio[j]
+        cn0 = 10 * math.log10(rj / tcoh / 1.0) if rj > 0 else (-math.inf if rj == 0 else math.nan)
+        row, col = rows[j], cols[j]
+        delay = fmdbzp_column_to_tau(col % spc, nb, bsz)
+        code_phase = jl_mod(-delay * fc / fs, code_length)
+        doppler = float(plan.doppler_freqs[row])
+        if subsample_interpolation:
+            code_phase, doppler = _interpolate(plan, peak[j], noise[j], nbrs[j][0:2], nbrs[j][2:4], delay, code_phase,
+                                               doppler)
+        r = AcquisitionResults(
+            system=system, prn=p, sampling_frequency=fs, carrier_doppler=doppler, code_phase=code_phase,
+            secondary_code_phase=None, CN0=cn0, noise_power=noise[j], peak_to_noise_ratio=ptn[j],
+            num_noncoherent_integrations=plan.num_noncoherent_accumulations,
+            power_bins=bufs[j] if bufs is not None else None, dopplers=plan.doppler_freqs, num_blocks=nb,
+            block_size=bsz, num_secondary_rotations=R)
+        r.engine_stats = dict(peak=peaks[j], row=row, col=col, noise=noises[j], nbr=nbrs[j], noise_sum=nsums[j],
+                              noise_cells=ncells[j])
+        if thr is not None and float(ptn[j]) > thr:
             pending.append(r)
-        st = stats[j]
-        r.engine_stats = dict(peak=float(st.peak_power), row=int(st.peak_row), col=int(st.peak_col),
-                              noise=float(st.noise_power), nbr=[float(st.code_left), float(st.code_right),
-                                                                float(st.dop_left), float(st.dop_right)],
-                              noise_sum=float(st.noise_sum), noise_cells=int(st.noise_cells))
         results.append(r)
     if pending:
         # `_estimate_secondary_code_phase` (src/acquire.jl:503-538) on the device
-        sysm, spc, fs = plan.system, plan.samples_per_code, plan.sampling_freq
+        sysm = plan.system
         items = [(sysm.gen_code(spc, r.prn, fs, sysm.get_code_frequency(), r.code_phase),
-                  np.array([sysm.secondary_value(r.prn, k) for k in range(plan.num_secondary_rotations)],
-                           dtype=np.int8), r.carrier_doppler) for r in pending]
+                  np.array([sysm.secondary_value(r.prn, k) for k in range(R)], dtype=np.int8), r.carrier_doppler)
+                 for r in pending]
         flags = _native.FLAG_DEVICE_SIGNAL if sv.device else 0
         phases = plan.native.secondary_phase(sv.ptr, _native.ELTYPE[sv.kind], sv.n, float(interm_freq), flags, items)
         for r, ph in zip(pending, phases):
