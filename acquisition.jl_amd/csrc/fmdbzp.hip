@@ -963,9 +963,14 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
         hinv = (float)std::ldexp(1.0, e);
     }
 
-    // fork the lanes off the plan's stream (signal FFT cache, request list)
-    HIP_TRY(hipEventRecord(P->ev_fork, P->stream));
-    for (int l = 0; l < P->nlanes; l++) HIP_TRY(hipStreamWaitEvent(P->lane[l], P->ev_fork, 0));
+    // fork the lanes off the plan's stream (signal FFT cache, request list);
+    // a single lane runs on the plan's stream itself, which saves the two
+    // cross-stream dependencies (about 10 us each on the device)
+    const bool forked = P->nlanes > 1;
+    if (forked) {
+        HIP_TRY(hipEventRecord(P->ev_fork, P->stream));
+        for (int l = 0; l < P->nlanes; l++) HIP_TRY(hipStreamWaitEvent(P->lane[l], P->ev_fork, 0));
+    }
     const size_t tile_per_prn = (size_t)nb * ndop * P->tp;
     const size_t acc_per_prn = P->acc_ps;
     const bool dev_out = (flags & FMDBZP_FLAG_DEVICE_OUTPUT) != 0;
@@ -977,7 +982,7 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
             const int lo = l * per_lane;
             const int cnt = nbat - lo < per_lane ? nbat - lo : per_lane;
             if (cnt <= 0) continue;
-            hipStream_t st = P->lane[l];
+            hipStream_t st = forked ? P->lane[l] : P->stream;
             const int32_t* req = P->d_req + b0 + lo;
             float2* tile = P->d_tile + (size_t)lo * tile_per_prn;
             float* acc = P->d_acc + (size_t)lo * acc_per_prn;
@@ -1083,7 +1088,7 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
         }
     }
     // join the lanes back into the plan's stream
-    for (int l = 0; l < P->nlanes; l++) {
+    for (int l = 0; forked && l < P->nlanes; l++) {
         HIP_TRY(hipEventRecord(P->ev_join[l], P->lane[l]));
         HIP_TRY(hipStreamWaitEvent(P->stream, P->ev_join[l], 0));
     }
