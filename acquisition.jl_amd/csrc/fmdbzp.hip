@@ -115,8 +115,9 @@ struct TimedEvent {
 // PRN lanes: a batch of PRNs can be split over up to kLanes streams, so the
 // block correlation of one lane overlaps the column stage of the other.
 // FMDBZP_LANES sets the count (default 1): on the 32-PRN L1 C/A workload two
-// lanes gain about 1.5%, as each kernel runs about half as fast while they
-// overlap, and per-launch timings then include the other lane's work.
+// lanes gain under 1%, as each kernel runs about half as fast while they
+// overlap (they compete for the same CU slots), and per-launch timings then
+// include the other lane's work.
 constexpr int kLanes = 2;
 
 static int lanes_requested()

@@ -416,7 +416,8 @@ k_column_c160(ColumnArgs A, float hinv)
                     const __half2* Th = (const __half2*)A.tile + (((size_t)p * A.nb + r) * N) * A.tp + lag0;
 #pragma unroll
                     for (int m = 0; m < R0; m++) {
-                        const uint32_t raw = __builtin_nontemporal_load((const uint32_t*)&Th[(size_t)(j + M0 * m) * A.tp + t]);
+                        const uint32_t raw = __builtin_nontemporal_load(
+                            (const uint32_t*)((const char*)Th + (uint32_t)((j + M0 * m) * A.tp + t) * (uint32_t)sizeof(__half2)));
                         const float2 u = __half22float2(*(const __half2*)&raw);
                         v[m] = make_float2(u.x * hinv, u.y * hinv);
                     }
