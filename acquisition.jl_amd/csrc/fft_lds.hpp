@@ -642,6 +642,8 @@ __device__ __forceinline__ void stage_static_inplace_pair(float2* a, const float
     constexpr int TSTRIDE = N / (NS * R);
     static_assert(M % 16 == 0 && lds_group_ok<true, NS, R>(), "grouped padded addressing");
     float2 v[ITERS][2][R];
+    // twiddles are loaded with the inputs, so their latency overlaps the barrier
+    float2 w[ITERS][R];
 #pragma unroll
     for (int it = 0; it < ITERS; it++) {
         const int j = tid + it * NTHR;
@@ -651,6 +653,11 @@ __device__ __forceinline__ void stage_static_inplace_pair(float2* a, const float
             for (int u = 0; u < 2; u++)
 #pragma unroll
                 for (int r = 0; r < R; r++) v[it][u][r] = src[u * BST + r * lds_group_step<true, M>()];
+            if (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; r++) w[it][r] = twid<INV>(tw, r * k * TSTRIDE);
+            }
         }
     }
     __syncthreads();
@@ -662,9 +669,8 @@ __device__ __forceinline__ void stage_static_inplace_pair(float2* a, const float
             if (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    const float2 w = twid<INV>(tw, r * k * TSTRIDE);
-                    v[it][0][r] = cmulf(v[it][0][r], w);
-                    v[it][1][r] = cmulf(v[it][1][r], w);
+                    v[it][0][r] = cmulf(v[it][0][r], w[it][r]);
+                    v[it][1][r] = cmulf(v[it][1][r], w[it][r]);
                 }
             }
             bfly<R, INV>(v[it][0]);

@@ -840,6 +840,14 @@ k_column_rot2000(ColumnArgs A)
         stage_static<16, false, N, 1, 2, FMDBZP_NTHR>(RotSignIn{xr, m0, m1, magic}, LdsOut<true>{a, BST}, A.twD,
                                                       tid_pass);
         __syncthreads();
+        // last-stage twiddles, loaded ahead of the two in-place stages
+        float2 w3[IT3][5];
+#pragma unroll
+        for (int it = 0; it < IT3; it++) {
+            const int j = min(tid_pass + it * FMDBZP_NTHR, M3 - 1);
+#pragma unroll
+            for (int i = 1; i < 5; i++) w3[it][i] = twid<false>(A.twD, i * j);
+        }
         stage_static_inplace_pair<5, false, N, 16, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
         stage_static_inplace_pair<5, false, N, 80, FMDBZP_NTHR, BST>(a, A.twD, tid_pass);
 #pragma unroll
@@ -854,9 +862,8 @@ k_column_rot2000(ColumnArgs A)
                     for (int i = 0; i < 5; i++) v[u][i] = src[u * BST + i * lds_group_step<true, M3>()];
 #pragma unroll
                 for (int i = 1; i < 5; i++) {
-                    const float2 w = twid<false>(A.twD, i * j);
-                    v[0][i] = cmulf(v[0][i], w);
-                    v[1][i] = cmulf(v[1][i], w);
+                    v[0][i] = cmulf(v[0][i], w3[it][i]);
+                    v[1][i] = cmulf(v[1][i], w3[it][i]);
                 }
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
