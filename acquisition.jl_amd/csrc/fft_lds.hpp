@@ -590,6 +590,8 @@ __device__ __forceinline__ void stage_static_inplace(float2* a, const float2* __
     constexpr int TSTRIDE = N / (NS * R);
     constexpr int ITERS = (TOTAL + NTHR - 1) / NTHR;
     float2 v[ITERS][R];
+    // twiddles are loaded with the inputs, so their latency overlaps the barrier
+    float2 w[ITERS][R];
 #pragma unroll
     for (int it = 0; it < ITERS; it++) {
         const int idx = tid + it * NTHR;
@@ -603,6 +605,11 @@ __device__ __forceinline__ void stage_static_inplace(float2* a, const float2* __
 #pragma unroll
                 for (int r = 0; r < R; r++) v[it][r] = a[t * BST + lds_pad(j + r * M)];
             }
+            if (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; r++) w[it][r] = twid<INV>(tw, r * k * TSTRIDE);
+            }
         }
     }
     __syncthreads();
@@ -614,7 +621,7 @@ __device__ __forceinline__ void stage_static_inplace(float2* a, const float2* __
             const int k = j % NS;
             if (NS > 1) {
 #pragma unroll
-                for (int r = 1; r < R; r++) v[it][r] = cmulf(v[it][r], twid<INV>(tw, r * k * TSTRIDE));
+                for (int r = 1; r < R; r++) v[it][r] = cmulf(v[it][r], w[it][r]);
             }
             bfly<R, INV>(v[it]);
             const int base = (j - k) * R + k;
