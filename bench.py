@@ -262,7 +262,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the engine runs on a single MI355X; see the sharding notes in DESIGN.md)", file=sys.stderr)
+              "(the engine runs on a single MI355X; across GPUs the search would shard by PRN and request)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "c5":
