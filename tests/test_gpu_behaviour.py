@@ -2,14 +2,16 @@
 reference's `acquire` tests (test/acquire.jl) with the same signals and bounds:
 detection and code phase of a strong and a weak signal, subsample
 interpolation against the grid, N_nc 1 against N_nc 2 on the same coherent
-segment, the sign-search path's Doppler axis, and the searches with more than
-320 Doppler bins. The noise comes from this package's seeded generator, not
-Julia's RNG, so the assertions are the reference's tolerances, not values.
+segment, the sign-search path's Doppler axis, the searches with more than
+320 Doppler bins, interpolation with and without stored surfaces, and the
+reuse of the plan-owned surface buffer. The noise comes from this package's
+seeded generator, not Julia's RNG, so the assertions are the reference's
+tolerances, not values.
 """
 import pytest
 
-from acquisition import (BATCH_FFT_THRESHOLD, GPSL1CA, acquire, acquire_, cfar_threshold, get_num_cells,
-                         is_detected, plan_acquire)
+from acquisition import (BATCH_FFT_THRESHOLD, GPSL1CA, GPSL5I, acquire, acquire_, cfar_threshold,
+                         get_num_cells, is_detected, plan_acquire)
 from acquisition.signals import generate_test_signal
 
 pytestmark = pytest.mark.gpu
@@ -129,5 +131,59 @@ def test_more_than_320_doppler_bins_simple_path():
         r = acquire_(plan, g["signal"].astype("complex64"), [1], interm_freq=0.0)[0]
         assert is_detected(r)
         assert abs(r.code_phase - 100.0) < 1.0
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("path", ["simple", "rotation"])
+def test_interpolation_with_and_without_stored_surface(path):
+    """test/acquire.jl:388-433: the interpolated code phase and Doppler (and
+    the secondary phase on the rotation path) do not depend on
+    store_power_bins."""
+    print(f"behaviour: interpolation, {path} path, with and without surfaces", flush=True)
+    if path == "simple":
+        sysm, fs, prn = GPSL1CA(), FS, 2
+        g = generate_test_signal(sysm, prn, num_samples=2048, doppler=250.0, code_phase=300.7, sampling_freq=fs,
+                                 interm_freq=0.0, CN0=45, seed=5)
+        plan = plan_acquire(sysm, fs, [prn])
+    else:
+        sysm, fs, prn = GPSL5I(), 10.24e6, 1
+        g = generate_test_signal(sysm, prn, num_samples=10 * 10240, doppler=850.0, code_phase=5115.3, sampling_freq=fs,
+                                 interm_freq=0.0, CN0=45, seed=6)
+        plan = plan_acquire(sysm, fs, [prn], num_coherently_integrated_code_periods=10)
+    try:
+        if path == "rotation":
+            assert plan.num_secondary_rotations == 10
+        sig = g["signal"].astype("complex64")
+        a = acquire_(plan, sig, prn, interm_freq=0.0, subsample_interpolation=True)
+        b = acquire_(plan, sig, prn, interm_freq=0.0, subsample_interpolation=True, store_power_bins=True)
+        if path == "rotation":
+            assert is_detected(a)
+            assert a.secondary_code_phase == b.secondary_code_phase
+        assert abs(a.code_phase - b.code_phase) <= 1e-3
+        assert abs(a.carrier_doppler - b.carrier_doppler) < 0.01
+    finally:
+        plan.close()
+
+
+def test_multistep_interpolation_and_stored_surface():
+    """test/acquire.jl:435-460: N_nc 4 with interpolation and a stored
+    surface; a second run stores the same surface in the same plan-owned
+    buffer."""
+    print("behaviour: multistep stored surface", flush=True)
+    true_cp = 700.4
+    g = _sig(1, 4 * 2048, doppler=500.0, code_phase=true_cp, CN0=42, seed=8)
+    sig = g["signal"].astype("complex64")
+    plan = plan_acquire(GPSL1CA(), FS, [1], num_noncoherent_accumulations=4)
+    try:
+        r = acquire_(plan, sig, 1, interm_freq=0.0, subsample_interpolation=True, store_power_bins=True)
+        assert is_detected(r)
+        assert abs(r.code_phase - true_cp) <= 1.0
+        assert r.power_bins.dtype.name == "float32"
+        assert r.power_bins.shape == (plan.num_doppler_bins, plan.samples_per_code_eff)
+        first = r.power_bins.copy()
+        r2 = acquire_(plan, sig, 1, interm_freq=0.0, subsample_interpolation=True, store_power_bins=True)
+        assert (r2.power_bins == first).all()
+        assert r2.power_bins is plan.result_buffers[0]
     finally:
         plan.close()
