@@ -3,8 +3,9 @@ device, restating the reference's end-to-end checks with the same signals
 and bounds (test/secondary_code_search.jl, test/noncoherent_integration.jl):
 rotation discovery at every NH10 starting phase with the exact secondary
 phase, the coherent gain over the opt-out, the worst-case code phase, the
-CFAR cell count with the rotation expansion, and the 4-step bit-edge search on
-a strong signal.
+CFAR cell count with the rotation expansion, the rotation search's Doppler
+axis, two data symbols under the rotation search, data-bit transitions, and
+the 4-step bit-edge search on a strong signal.
 """
 import math
 
@@ -146,3 +147,69 @@ def test_bit_edge_search_keeps_strong_signal():
             assert min(diff, alias - diff) < plan.doppler_step
         finally:
             plan.close()
+
+
+def test_rotation_doppler_axis():
+    """test/secondary_code_search.jl:177-206: the rotation search reports the
+    absolute Doppler within one bin (not half the band away)."""
+    print("secondary: rotation Doppler", flush=True)
+    fs, dop, cp = 12e6, 1000.0, 1234.5
+    g = generate_test_signal(GPSL5I(), 1, num_samples=10 * 12000, doppler=dop, code_phase=cp, sampling_freq=fs,
+                             interm_freq=0.0, CN0=45)
+    plan = plan_acquire(GPSL5I(), fs, [1], num_coherently_integrated_code_periods=10, num_noncoherent_accumulations=1)
+    try:
+        assert plan.num_secondary_rotations > 1
+        r = acquire_(plan, g["signal"], 1, interm_freq=0.0)
+        assert is_detected(r)
+        assert abs(r.code_phase - cp) <= 1.0
+        assert abs(r.carrier_doppler - dop) < plan.doppler_step
+    finally:
+        plan.close()
+
+
+def test_rotation_with_two_data_symbols():
+    """test/secondary_code_search.jl:208-258: N_coh 20 on L5I (two 10 ms
+    symbols, a data-bit flip between them) runs the data-bit search and the
+    rotation search together; detection, code phase, Doppler and secondary
+    phase 0 are recovered."""
+    print("secondary: N_coh 20, data flip", flush=True)
+    fs, dop, cp, N, L = 12e6, 1000.0, 1234.5, 20, 10
+    sysm = GPSL5I()
+    spc = math.ceil(sysm.code_length / sysm.code_frequency * fs)
+    primary = sysm.gen_code(spc, 1, fs, sysm.code_frequency, cp).astype(np.float64)
+    sig = np.concatenate([sysm.secondary_value(1, k % L) * (1.0 if k < L else -1.0) * primary for k in range(N)])
+    sig = (sig * np.exp(1j * 2 * np.pi * dop / fs * np.arange(sig.size))).astype(np.complex64)
+    plan = plan_acquire(sysm, fs, [1], num_coherently_integrated_code_periods=N, num_noncoherent_accumulations=1)
+    try:
+        assert plan.num_secondary_rotations == L and plan.num_data_bits == N // L
+        r = acquire_(plan, sig, 1, interm_freq=0.0)
+        assert is_detected(r)
+        assert abs(r.code_phase - cp) <= 1.0
+        assert abs(r.carrier_doppler - dop) < plan.doppler_step
+        assert r.secondary_code_phase == 0
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("ncoh,flip", [(40, (20, 40)), (60, (20, 40))])
+def test_data_bit_transitions_recovered(ncoh, flip):
+    """test/noncoherent_integration.jl:602-702: at 0 Hz and 60 dB-Hz, a signal
+    with data-bit transitions (the bit after 20 ms negated, of 2 or 3 bits)
+    reaches the peak of the same signal without transitions within 10 %."""
+    print(f"secondary: bit transitions, N_coh {ncoh}", flush=True)
+    fs = 2.048e6
+    g = generate_test_signal(GPSL1CA(), 1, num_samples=ncoh * 2048, doppler=0.0, code_phase=0.0, sampling_freq=fs,
+                             interm_freq=0.0, CN0=60, seed=42)
+    clean = g["signal"].astype(np.complex64)
+    flipped = clean.copy()
+    flipped[flip[0] * 2048:flip[1] * 2048] *= -1
+    plan = plan_acquire(GPSL1CA(), fs, [1], min_doppler_coverage=10_000.0, num_coherently_integrated_code_periods=ncoh,
+                        bit_edge_search_steps=1, num_noncoherent_accumulations=1)
+    try:
+        assert plan.num_data_bits == ncoh // 20
+        p0 = float(np.max(acquire_(plan, clean, [1], interm_freq=0.0, store_power_bins=True)[0].power_bins))
+        p1 = float(np.max(acquire_(plan, flipped, [1], interm_freq=0.0, store_power_bins=True)[0].power_bins))
+    finally:
+        plan.close()
+    print(f"N_coh {ncoh}: peak with transitions / without {p1 / p0:.4f}")
+    assert abs(p1 / p0 - 1) <= 0.1
