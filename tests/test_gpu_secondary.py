@@ -4,8 +4,9 @@ and bounds (test/secondary_code_search.jl, test/noncoherent_integration.jl):
 rotation discovery at every NH10 starting phase with the exact secondary
 phase, the coherent gain over the opt-out, the worst-case code phase, the
 CFAR cell count with the rotation expansion, the rotation search's Doppler
-axis, two data symbols under the rotation search, data-bit transitions, and
-the 4-step bit-edge search on a strong signal.
+axis, two data symbols under the rotation search, data-bit transitions, the
+4-step bit-edge search on a strong signal, and the rotation search against one
+10 ms transform of the combined 102300-chip code.
 """
 import math
 
@@ -14,6 +15,7 @@ import pytest
 
 from acquisition import GPSL1CA, GPSL5I, acquire, acquire_, cfar_threshold, get_num_cells, is_detected, plan_acquire
 from acquisition.signals import generate_test_signal
+from acquisition.systems import GNSSSystem
 
 pytestmark = pytest.mark.gpu
 
@@ -213,3 +215,43 @@ def test_data_bit_transitions_recovered(ncoh, flip):
         plan.close()
     print(f"N_coh {ncoh}: peak with transitions / without {p1 / p0:.4f}")
     assert abs(p1 / p0 - 1) <= 0.1
+
+
+class _LongL5I(GNSSSystem):
+    """GPS L5I re-presented as one 102300-chip primary code (10 primary
+    periods times NH10) with no secondary code, so that an N_coh 1 search runs
+    one 10 ms column transform (test/secondary_code_search.jl:1-29)."""
+
+    def __init__(self):
+        l5 = GPSL5I()
+        super().__init__("LongL5I", 10 * l5.code_length, l5.get_code_frequency(), l5.get_center_frequency())
+        self._l5 = l5
+
+    def _make_chips(self, prn):
+        l5 = self._l5
+        return np.concatenate([l5.primary_chips(prn) * l5.secondary_value(prn, k) for k in range(10)]).astype(np.int8)
+
+
+def test_rotation_matches_full_10ms_transform():
+    """test/secondary_code_search.jl:388-471: on a noiseless NH10 signal the
+    rotation search's peak reaches more than 0.95 of the one-long-code 10 ms
+    search's peak, on the 1 kHz sub-block grid (1000 Hz) and off it (100 Hz,
+    500 Hz)."""
+    print("secondary: rotation vs one 10 ms transform", flush=True)
+    fs, ncoh, spc = 12e6, 10, 12000
+    l5, long = GPSL5I(), _LongL5I()
+    code = l5.gen_code(ncoh * spc, 1, fs, l5.get_code_frequency(), 0.0).astype(np.float64)
+    rot = plan_acquire(l5, fs, [1], num_coherently_integrated_code_periods=ncoh, num_noncoherent_accumulations=1)
+    lng = plan_acquire(long, fs, [1], num_coherently_integrated_code_periods=1, num_noncoherent_accumulations=1)
+    try:
+        assert rot.num_doppler_bins == lng.num_doppler_bins == 150
+        assert rot.doppler_step == pytest.approx(lng.doppler_step)
+        for dop in (1000.0, 100.0, 500.0):
+            sig = (np.exp(1j * 2 * np.pi * dop / fs * np.arange(code.size)) * code).astype(np.complex64)
+            pr = float(np.max(acquire_(rot, sig, 1, interm_freq=0.0, store_power_bins=True).power_bins))
+            pl = float(np.max(acquire_(lng, sig, 1, interm_freq=0.0, store_power_bins=True).power_bins))
+            print(f"{dop:.0f} Hz: rotation / long {pr / pl:.4f}")
+            assert pr / pl > 0.95
+    finally:
+        rot.close()
+        lng.close()
