@@ -831,6 +831,13 @@ template <class TileT>
 void launch_corr(fmdbzp_plan* P, dim3 g, hipStream_t st, const float2* sf, const int32_t* req, TileT* tile, int nb,
                  int bs, int ndop, float hscale)
 {
+    // Below about 2048 workgroups (under three waves of the device's
+    // workgroup slots) split each row's column-block passes over g.z
+    // workgroups, at most one pass each.
+    const int nt = (P->corr_variant == 2 || P->corr_variant == 3) ? 2 : 1;
+    const int npass = (nb + nt - 1) / nt;
+    const int wgs = (int)(g.x * g.y);
+    g.z = wgs >= 2048 ? 1 : std::min(npass, (2048 + wgs - 1) / wgs);
     if (P->corr_variant == 4)
         k_corr_tile<1, true, 2500, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
             sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);

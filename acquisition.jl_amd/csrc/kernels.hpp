@@ -231,7 +231,7 @@ struct LagOut {
 // reused for all nb column blocks r, NT blocks per pass. ONEBUF: in-place
 // stages, one LDS buffer per concurrent transform; else two ping-pong buffers
 // (NT = 1).
-// grid (ndop, batch)
+// grid (ndop, batch, split)
 // STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass, with the
 // 1/F factor taken from the code spectra; STATIC = 2500: 10 x 10 x 5 x 5 (the
 // E1-like search at 25 MHz). inv_f is 1 when the code spectra carry 1/F.
@@ -247,7 +247,15 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
     float2* srow = lds;
     float2* a = lds + F;
     const int lbk = xcd_logical_block();
-    const int p = lbk / gridDim.x, k = lbk - p * gridDim.x;
+    const int k = lbk % gridDim.x, rest = lbk / gridDim.x;
+    const int p = rest % gridDim.y, sp = rest / gridDim.y;
+    // gridDim.z > 1 splits a row's column blocks over several workgroups (small
+    // searches, where ndop x PRNs workgroups would not fill the device); this
+    // workgroup takes passes [sp * chunk, (sp + 1) * chunk)
+    const int npass = (nb + NT - 1) / NT;
+    const int chunk = (npass + gridDim.z - 1) / gridDim.z;
+    const int r_begin = sp * chunk * NT, r_end = min(nb, (sp + 1) * chunk * NT);
+    if (r_begin >= r_end) return;  // uniform over the workgroup, before any barrier
     const int tid = threadIdx.x;
     const int prn = req[p];
     const float2* S = sigfft_step + (size_t)k * F;
@@ -255,7 +263,7 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
     const float2* Pp = prnconj + (size_t)prn * nb * F;
     const int kb = k % nb;
 #pragma unroll 1
-    for (int r0 = 0; r0 < nb; r0 += NT) {
+    for (int r0 = r_begin; r0 < r_end; r0 += NT) {
         __syncthreads();  // srow ready / previous transforms done with the buffers
         const int nt = min(NT, nb - r0);
         int b0 = kb + r0;
