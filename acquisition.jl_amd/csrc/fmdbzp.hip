@@ -189,8 +189,8 @@ struct fmdbzp_plan {
     bool tile_half = false;      // FMDBZP_PREC_TILE_FP16: FP16 coherent tile
     bool rot_direct = false;     // rotation kernel writes transform outputs straight to the accumulator
     bool rot2000 = false;        // direct rotation at ndop 2000: compile-time kernel, one column per workgroup
-    int corr_variant = 0;        // k_corr_tile: 4 = compile-time F = 2500, 3 = compile-time F = 2048,
-                                 // 2 = two in-place transforms per
+    int corr_variant = 0;        // k_corr_tile: 5 = compile-time F = 500, 4 = compile-time F = 2500,
+                                 // 3 = compile-time F = 2048, 2 = two in-place transforms per
                                  // pass, 1 = one, 0 = ping-pong
     size_t col_lds = 0;
     size_t fft_lds = 0;
@@ -493,6 +493,7 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
     // register bound), else two ping-pong buffers
     if (F == 2048) P->corr_variant = 3;  // compile-time 16 x 16 x 8 transforms, two per pass, unscaled
     else if (F == 2500) P->corr_variant = 4;  // compile-time 10 x 10 x 5 x 5, one per pass
+    else if (F == 500) P->corr_variant = 5;   // compile-time 4 x 5 x 5 x 5, two per pass
     else if (fft_inplace_ok(P->pf) && 2 * F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 2;
     else if (fft_inplace_ok(P->pf) && F <= FMDBZP_CORR_MAXV * FMDBZP_NTHR) P->corr_variant = 1;
     else P->corr_variant = 0;
@@ -577,6 +578,8 @@ int fmdbzp_plan_create(const fmdbzp_plan_desc* desc, int32_t nprn, const int8_t*
         (rc = set_lds_attr((const void*)k_corr_tile<1, false>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<2, true, 2048, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true, 2500>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<2, true, 500>, P->corr_lds)) ||
+        (rc = set_lds_attr((const void*)k_corr_tile<2, true, 500, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true, 2500, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<2, true, 0, __half2>, P->corr_lds)) ||
         (rc = set_lds_attr((const void*)k_corr_tile<1, true, 0, __half2>, P->corr_lds)) ||
@@ -834,11 +837,14 @@ void launch_corr(fmdbzp_plan* P, dim3 g, hipStream_t st, const float2* sf, const
     // Below about 2048 workgroups (under three waves of the device's
     // workgroup slots) split each row's column-block passes over g.z
     // workgroups, at most one pass each.
-    const int nt = (P->corr_variant == 2 || P->corr_variant == 3) ? 2 : 1;
+    const int nt = (P->corr_variant == 2 || P->corr_variant == 3 || P->corr_variant == 5) ? 2 : 1;
     const int npass = (nb + nt - 1) / nt;
     const int wgs = (int)(g.x * g.y);
     g.z = wgs >= 2048 ? 1 : std::min(npass, (2048 + wgs - 1) / wgs);
-    if (P->corr_variant == 4)
+    if (P->corr_variant == 5)
+        k_corr_tile<2, true, 500, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
+            sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);
+    else if (P->corr_variant == 4)
         k_corr_tile<1, true, 2500, TileT><<<g, FMDBZP_NTHR, P->corr_lds, st>>>(
             sf, P->d_prnconj, req, tile, P->pf, P->d_twF, nb, bs, P->tp, ndop, P->tile_scale, hscale);
     else if (P->corr_variant == 3)

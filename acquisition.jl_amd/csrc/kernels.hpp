@@ -234,7 +234,9 @@ struct LagOut {
 // grid (ndop, batch, split)
 // STATIC = 2048: compile-time 16 x 16 x 8 transform, two per pass, with the
 // 1/F factor taken from the code spectra; STATIC = 2500: 10 x 10 x 5 x 5 (the
-// E1-like search at 25 MHz). inv_f is 1 when the code spectra carry 1/F.
+// E1-like search at 25 MHz); STATIC = 500: 4 x 5 x 5 x 5, two per pass (GPS
+// L1 C/A at 5 MHz, the radix order of the runtime plan for 500). inv_f is 1
+// when the code spectra carry 1/F.
 template <int NT, bool ONEBUF, int STATIC = 0, class TileT = float2>
 __global__ void __launch_bounds__(FMDBZP_NTHR)
 k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ prnconj,
@@ -275,6 +277,8 @@ k_corr_tile(const float2* __restrict__ sigfft_step, const float2* __restrict__ p
             fft3_static<true, 2048, NT, FMDBZP_NTHR, 16, 16, 8>(a, twF, tid, in, out);
         else if constexpr (STATIC == 2500)
             fft4_static<true, 2500, NT, FMDBZP_NTHR, 10, 10, 5, 5>(a, twF, tid, in, out);
+        else if constexpr (STATIC == 500)
+            fft4_static<true, 500, NT, FMDBZP_NTHR, 4, 5, 5, 5>(a, twF, tid, in, out);
         else if constexpr (ONEBUF)
             fft_run_1buf<true, false, FMDBZP_CORR_MAXV>(a, pf, twF, nt, span, tid, FMDBZP_NTHR, in, out);
         else
