@@ -23,9 +23,11 @@ from .plan import AcquisitionPlan, ArgumentError, plan_acquire
 f32 = np.float32
 
 
-@dataclass
+@dataclass(slots=True)
 class AcquisitionResults:
-    """Results of one PRN (src/Acquisition.jl:81-109). Frequencies in Hz."""
+    """Results of one PRN (src/Acquisition.jl:81-109). Frequencies in Hz.
+    `engine_stats` exposes the engine's per-PRN statistics behind the result
+    (peak, row, col, noise, code and Doppler neighbours, noise sum and cells)."""
 
     system: Any
     prn: int
@@ -42,6 +44,15 @@ class AcquisitionResults:
     num_blocks: int
     block_size: int
     num_secondary_rotations: int
+    _stats: Optional[tuple] = None  # (peak, row, col, noise, nbr[4], noise_sum, noise_cells)
+
+    @property
+    def engine_stats(self) -> Optional[dict]:
+        if self._stats is None:
+            return None
+        peak, row, col, noise, nbr, noise_sum, noise_cells = self._stats
+        return dict(peak=peak, row=row, col=col, noise=noise, nbr=list(nbr), noise_sum=noise_sum,
+                    noise_cells=noise_cells)
 
     def __repr__(self):
         return (f"AcquisitionResults: PRN {self.prn}, CN0 = {round(self.CN0, 2)} dB-Hz, "
@@ -330,6 +341,7 @@ def _results_from_stats(plan, stats, prns, sv, interm_freq, subsample_interpolat
     if R > 1:
         thr = cfar_threshold(0.01, plan.num_doppler_bins * spc * R,
                              num_noncoherent_integrations=plan.num_noncoherent_accumulations)
+    nnc, dfreqs = plan.num_noncoherent_accumulations, plan.doppler_freqs
     results, pending = [], []
     for j, p in enumerate(prns):
         rj = ratio[j]
@@ -341,14 +353,10 @@ def _results_from_stats(plan, stats, prns, sv, interm_freq, subsample_interpolat
         if subsample_interpolation:
             code_phase, doppler = _interpolate(plan, peak[j], noise[j], nbrs[j][0:2], nbrs[j][2:4], delay, code_phase,
                                                doppler)
-        r = AcquisitionResults(
-            system=system, prn=p, sampling_frequency=fs, carrier_doppler=doppler, code_phase=code_phase,
-            secondary_code_phase=None, CN0=cn0, noise_power=noise[j], peak_to_noise_ratio=ptn[j],
-            num_noncoherent_integrations=plan.num_noncoherent_accumulations,
-            power_bins=bufs[j] if bufs is not None else None, dopplers=plan.doppler_freqs, num_blocks=nb,
-            block_size=bsz, num_secondary_rotations=R)
-        r.engine_stats = dict(peak=peaks[j], row=row, col=col, noise=noises[j], nbr=nbrs[j], noise_sum=nsums[j],
-                              noise_cells=ncells[j])
+        # positional: the field order of AcquisitionResults
+        r = AcquisitionResults(system, p, fs, doppler, code_phase, None, cn0, noise[j], ptn[j], nnc,
+                               bufs[j] if bufs is not None else None, dfreqs, nb, bsz, R,
+                               (peaks[j], row, col, noises[j], nbrs[j], nsums[j], ncells[j]))
         if thr is not None and float(ptn[j]) > thr:
             pending.append(r)
         results.append(r)
