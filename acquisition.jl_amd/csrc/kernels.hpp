@@ -209,8 +209,8 @@ struct LagOut {
     {
         if (i < bs && t < nt) {
             const float sx = SCALE ? v.x * inv_f : v.x, sy = SCALE ? v.y * inv_f : v.y;
-            // 32-bit byte offset from the pass's base (the tile of one PRN is
-            // below 2^31 bytes, checked at plan creation)
+            // 32-bit byte offset from the pass's base (the two column blocks
+            // of one pass span below 2^31 bytes, checked at plan creation)
             char* dst = (char*)out + ((uint32_t)t * tstride + (uint32_t)i) * (uint32_t)sizeof(TileT);
             // the tile is written once and read by the next kernel: nontemporal stores
             if constexpr (sizeof(TileT) == sizeof(float2)) {
