@@ -72,6 +72,18 @@ def test_l1ca_5mhz_32prn_streamed():
     assert min(d, 1023 - d) < 1.0
 
 
+def test_l1ca_5mhz_16prn_split_correlation_grid():
+    """PRNs 1..16 of the 5 MHz scene: 20 x 16 = 320 correlation workgroups, so
+    each row's 10 column-block passes are split over 7 workgroups of 2 passes,
+    the last two of which have nothing to do and return early."""
+    sc = l1ca_5mhz_scene()
+    prns = list(range(1, 17))
+    plan, res, ores = run_case(sc.system, sc.sampling_freq, prns, sc.signal, interp=True, label="c1-16")
+    prn = sc.truth["prn"]
+    if prn in prns:
+        assert is_detected(res[prns.index(prn)])
+
+
 def test_multistep_simple_with_drift():
     """N_coh 20, N_nc 8 at 2.048 MHz: the code-drift shift is non-zero at later steps."""
     sysm = GPSL1CA()
