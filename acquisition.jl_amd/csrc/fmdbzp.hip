@@ -166,6 +166,7 @@ struct fmdbzp_plan {
     uint64_t* d_pbest_k = nullptr;
     int32_t* d_req = nullptr;
     size_t req_cap = 0;
+    int req_n = 0;           // entries of d_req that hold the last uploaded request list (h_req)
     fmdbzp_prn_stats* d_stats = nullptr;
     // pinned host staging, so copies stay asynchronous to the host
     int32_t* h_req = nullptr;
@@ -916,9 +917,16 @@ int acquire_enqueue(fmdbzp_plan* P, const void* signal, int32_t eltype, int64_t 
         HIP_TRY(hipHostMalloc((void**)&P->h_req, sizeof(int32_t) * nreq, hipHostMallocDefault));
         HIP_TRY(hipHostMalloc((void**)&P->h_stats, sizeof(fmdbzp_prn_stats) * nreq, hipHostMallocDefault));
         P->req_cap = (size_t)nreq;
+        P->req_n = 0;
     }
-    memcpy(P->h_req, prn_index, sizeof(int32_t) * nreq);
-    HIP_TRY(hipMemcpyAsync(P->d_req, P->h_req, sizeof(int32_t) * nreq, hipMemcpyHostToDevice, P->stream));
+    // A call that repeats the previous PRN list reuses the device copy: every
+    // call synchronizes before it returns, so no upload is still in flight.
+    if (P->req_n != nreq || memcmp(P->h_req, prn_index, sizeof(int32_t) * nreq) != 0) {
+        P->req_n = 0;
+        memcpy(P->h_req, prn_index, sizeof(int32_t) * nreq);
+        HIP_TRY(hipMemcpyAsync(P->d_req, P->h_req, sizeof(int32_t) * nreq, hipMemcpyHostToDevice, P->stream));
+        P->req_n = nreq;
+    }
     const size_t surf_cols = P->surf_bins > 0 ? (size_t)P->surf_bins : (size_t)spc_eff;
     if (store && !P->d_export && !(flags & FMDBZP_FLAG_DEVICE_OUTPUT))
         HIP_TRY(hipMalloc((void**)&P->d_export, sizeof(float) * (size_t)ndop * surf_cols * kLanes));
