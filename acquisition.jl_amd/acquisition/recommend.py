@@ -218,8 +218,9 @@ def format_recommendations(rs: Sequence[SamplingFreqRecommendation]) -> str:
 # ---------------------------------------------------------------------------
 def _corr_variant(F: int) -> str:
     """The block-correlation kernel the engine runs for transform size F:
-    compile-time transforms for 2048 and 2500, the runtime-radix one otherwise."""
-    return {2048: "f2048", 2500: "f2500"}.get(F, "generic")
+    compile-time transforms for 500, 2048 and 2500, the runtime-radix one
+    otherwise."""
+    return {500: "f500", 2048: "f2048", 2500: "f2500"}.get(F, "generic")
 
 
 def _column_variant(ndop: int, bs: int) -> str:
@@ -275,8 +276,9 @@ def _column_work(spc: int, ndop: int) -> Tuple[float, float]:
 # per launch, corr = a0 + nprn·work·a1 and column = b0 + nprn·(cells·b1 +
 # work·b2), in seconds, with the work terms of _corr_work and _column_work.
 MI355X_COEFFS: Dict[str, Dict[str, Tuple[float, ...]]] = {
-    "corr": {"f2048": (1.950e-05, 3.025e-14), "f2500": (5.553e-05, 5.043e-14), "generic": (3.727e-05, 7.759e-14)},
-    "column": {"c160": (2.385e-06, 2.884e-12, 0.0), "generic": (6.260e-06, 5.109e-12, 0.0)},
+    "corr": {"f500": (7.132e-06, 7.629e-14), "f2048": (1.604e-05, 3.412e-14), "f2500": (1.425e-05, 5.456e-14),
+             "generic": (1.110e-05, 8.759e-14)},
+    "column": {"c160": (3.815e-07, 2.913e-12, 0.0), "generic": (6.567e-06, 5.121e-12, 0.0)},
 }
 
 

@@ -128,7 +128,8 @@ def test_show_forms():
 
 
 # -- engine cost model ----------------------------------------------------------------
-TRUE = GpuCostModel(corr={"f2048": (4e-6, 3.0e-14), "f2500": (4e-6, 6.0e-14), "generic": (6e-6, 8.0e-14)},
+TRUE = GpuCostModel(corr={"f500": (3e-6, 5.0e-14), "f2048": (4e-6, 3.0e-14), "f2500": (4e-6, 6.0e-14),
+                          "generic": (6e-6, 8.0e-14)},
                     column={"c160": (5e-6, 2.0e-12, 0.0), "generic": (5e-6, 1.5e-12, 6.0e-14)})
 
 
@@ -167,7 +168,7 @@ def _plan_geom(fs, N):
 def test_default_geometries_cover_every_variant():
     pts = _synthetic_points()
     from acquisition.recommend import _column_variant, _corr_variant
-    assert {_corr_variant(p["F"]) for p in pts} == {"f2048", "f2500", "generic"}
+    assert {_corr_variant(p["F"]) for p in pts} == {"f500", "f2048", "f2500", "generic"}
     assert {_column_variant(p["ndop"], p["spc"] // p["nb"]) for p in pts} == {"c160", "generic"}
 
 
@@ -190,12 +191,13 @@ def test_model_cost_ranks_candidates():
 
 def test_recorded_mi355x_table_matches_l1ca_profile():
     """The recorded MI355X coefficients predict the L1 C/A 16.368 MHz, 32 PRN,
-    N_coh 10 launches of the committed kernel profile (k_corr_tile 0.2548 ms,
-    k_column_c160 0.2642 ms per launch, mean over 8 noncoherent steps) within
-    15 %, and a sweep ranked under them comes back sorted by predicted cost."""
+    N_coh 10 launches of the committed kernel profile (k_corr_tile 0.2424 ms,
+    k_column_c160 0.2604 ms per launch, trace averages over 8 noncoherent
+    steps) within 15 %, and a sweep ranked under them comes back sorted by
+    predicted cost."""
     m = GpuCostModel.mi355x()
     tc, tk = m.predict_launches(16368, 16, 2048, 160, 32)
-    assert abs(tc / 0.2548e-3 - 1) < 0.15 and abs(tk / 0.2642e-3 - 1) < 0.15
+    assert abs(tc / 0.2424e-3 - 1) < 0.15 and abs(tk / 0.2604e-3 - 1) < 0.15
     rs = recommend_sampling_freqs(GPSL1CA(), fs_min=16.0e6, fs_max=16.5e6, num_coherently_integrated_code_periods=10,
                                   num_alternatives=50, cost_model=m)
     assert rs == sorted(rs, key=lambda r: (r.cost, r.num_doppler_bins_max_prime))
